@@ -1,0 +1,216 @@
+/*
+ * trax_hip.h -- C ABI of libtraxhip.so, the gfx950 (MI355X) track-seeding
+ * path: grid build -> hit pairs -> triplet prediction -> triplet filter, with
+ * the prefix sums between the stages.
+ *
+ * Every entry point takes plain device pointers and sizes, enqueues work on
+ * the given HIP stream (`void*` = hipStream_t; NULL = default stream) and
+ * returns an int status (TRAX_OK or a negative TRAX_ERR_*); the text of the
+ * last error is kept per context.  Nothing here allocates device memory
+ * except trax_ctx_create; callers own every buffer.  A context serialises its
+ * launches on one stream at a time (as the reference does with its one
+ * in-order OpenCL queue): use one context per concurrent stream.
+ *
+ * Each function replaces one piece of the reference's algorithm classes
+ * (paths relative to the reference's src/), cited at the declaration.
+ *
+ * Hit layout (src/datastructures/HitCollection.h:23, HitCollection.cpp:106-146):
+ * SoA columns x, y, z (float) and layer, detId, hitId, evt (uint32), ordered
+ * event-major then layer-ascending; per event an offset, per (event, layer) a
+ * hit count and an offset relative to the event.  evt[i] is the event's index
+ * in the batch and layer[i] the 1-based detector layer.
+ */
+#ifndef TRAX_HIP_H
+#define TRAX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRAX_ABI_VERSION 1
+
+#define TRAX_OK 0
+#define TRAX_ERR_INVALID (-1)  /* bad argument or configuration */
+#define TRAX_ERR_HIP (-2)      /* HIP runtime error */
+#define TRAX_ERR_NOMEM (-3)    /* device allocation failed (context creation) */
+#define TRAX_ERR_CAPACITY (-4) /* an output buffer was too small (see trax_ctx_flags) */
+
+/* Device status bits (trax_ctx_flags). */
+#define TRAX_FLAG_PAIR_OVERFLOW 0x1u
+#define TRAX_FLAG_CAND_OVERFLOW 0x2u
+#define TRAX_FLAG_TRACKLET_OVERFLOW 0x4u
+#define TRAX_FLAG_LAYOUT 0x8u
+
+#define TRAX_MAX_LAYERS 16
+#define TRAX_MAX_TRIPLETS 8
+#define TRAX_UNWRITTEN 0xFFFFFFFFu /* predictor slot the reference leaves unwritten */
+
+typedef struct trax_ctx trax_ctx;
+
+/* One row of the layer-triplet table; mirrors the columns of
+ * src/datastructures/TripletConfiguration.h:9-33 (configs/layerTriplets.xml). */
+typedef struct {
+  uint32_t layer1, layer2, layer3; /* layer1 must equal row index + 1 */
+  float dtheta_cut, sigma_z, dphi_cut, sigma_phi, tip_cut;
+  uint32_t pair_spread_z;
+  float z0; /* the reference parses it as uint (TripletConfiguration.cpp:91) */
+  uint32_t pair_spread_phi;
+  uint32_t n_candidates;
+} trax_layer_triplet;
+
+/* Grid binning (src/datastructures/Grid.h:5-39: MIN_Z=-300, MAX_Z=300,
+ * MIN_PHI=-pi, MAX_PHI=pi), per-layer radius bands (GeometrySupplement,
+ * index layer-1) and the layer triplets. */
+typedef struct {
+  uint32_t n_z, n_phi;
+  uint32_t n_layers; /* grid layers: detector layers 1..n_layers */
+  float min_radius[TRAX_MAX_LAYERS];
+  float max_radius[TRAX_MAX_LAYERS];
+  uint32_t n_triplets;
+  trax_layer_triplet triplets[TRAX_MAX_TRIPLETS];
+  /* minPt * GEV_C / (Bz * Q) * 100 with Bz = 3.8112 (TripletConfiguration.h:56-70);
+   * trax_min_radius_curvature() evaluates it the reference's way. */
+  float min_radius_curvature;
+} trax_seed_config;
+
+/* Hit columns.  Input: x..evt.  Grid-sorted output: all of them, plus
+ * orig_index (input index of each sorted hit) and phi (atan2(y, x)). */
+typedef struct {
+  float *x, *y, *z;
+  uint32_t *layer, *det_id, *hit_id, *evt;
+  uint32_t *orig_index;
+  float *phi;
+} trax_hits;
+
+/* Event layout, device arrays (EventSupplement.h / LayerSupplement.h). */
+typedef struct {
+  uint32_t n_events;
+  uint32_t n_hits;
+  const uint32_t *event_offset; /* [n_events] */
+  const uint32_t *layer_nhits;  /* [n_events * n_layers] */
+  const uint32_t *layer_offset; /* [n_events * n_layers], relative to the event */
+} trax_layout;
+
+/* Scratch of the count/store phases (device pointers, caller-owned). */
+typedef struct {
+  uint32_t *work_offsets;  /* >= max_events * n_triplets + 1 */
+  uint32_t *pair_counts;   /* >= max_hits + 1 */
+  uint32_t *cand_counts;   /* >= pair_capacity + 1 */
+  uint32_t *grid_counters; /* >= max_events * n_layers * cells, or NULL */
+  uint32_t max_events, max_hits;
+  uint64_t pair_capacity;
+} trax_workspace;
+
+/* Output buffers of the pair, prediction and filter stages.  Capacities are
+ * in elements; an overflow raises a TRAX_FLAG_* bit and the stage's count
+ * still reports the full size, so the caller can grow and rerun.
+ * pairs/cands hold uint2 records: pairs (inner, outer sorted-hit index),
+ * cands (pair index, third sorted-hit index).  Offsets have
+ * n_events * n_triplets + 1 entries.  totals[0..2] = pairs, candidates,
+ * triplets (device). */
+typedef struct {
+  uint32_t *pairs;
+  uint64_t pair_capacity;
+  uint32_t *pair_offsets;
+  uint32_t *cands;
+  uint64_t cand_capacity;
+  uint32_t *cand_offsets;
+  uint32_t *t1, *t2, *t3; /* TrackletHit1/2/3 (TrackletCollection.h:13-33) */
+  uint64_t tracklet_capacity;
+  uint32_t *tracklet_offsets;
+  uint32_t *totals;
+} trax_seed_outputs;
+
+/* ---------------------------------------------------------------- context */
+int trax_abi_version(void);
+int trax_ctx_create(int device, trax_ctx **out);
+int trax_ctx_destroy(trax_ctx *ctx);
+const char *trax_ctx_last_error(const trax_ctx *ctx);
+/* Synchronous read of the device status bits; clear != 0 resets them. */
+int trax_ctx_flags(trax_ctx *ctx, void *stream, uint32_t *flags_out, int clear);
+
+/* minRadiusCurvature(minPt) as TripletConfiguration.cpp:145-152 computes it. */
+float trax_min_radius_curvature(float min_pt);
+
+/* Carves one device allocation at `base` into a workspace for batches of at
+ * most max_events events, max_hits hits and pair_capacity pairs, and
+ * returns the bytes it needs (call with base = NULL to size it).  The grid
+ * counters are only laid out when (n_z+1)(n_phi+1) cells exceed 64 KiB of
+ * LDS. */
+uint64_t trax_workspace_layout(const trax_seed_config *cfg, uint32_t max_events, uint32_t max_hits,
+                               uint64_t pair_capacity, void *base, trax_workspace *ws);
+
+/* ------------------------------------------------------------- prefix sum */
+/* In-place exclusive uint32 scan (mod 2^32) of n elements; replaces
+ * PrefixSum::run(cl_mem, uint size, uint nThreads), PrefixSum.h:42.
+ * d_data must be 16-byte aligned. */
+int trax_prefix_sum(trax_ctx *ctx, uint32_t *d_data, uint64_t n, void *stream);
+
+/* Out-of-place variant whose length is read on the device: n = min(*d_n +
+ * n_add, n_max) (d_n may be NULL: n = n_max).  Graph-capture safe. */
+int trax_prefix_sum_dev(trax_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint64_t n_max,
+                        const uint32_t *d_n, uint32_t n_add, void *stream);
+
+/* ------------------------------------------------------------ grid build */
+/* GridBuilder::run(HitCollection&, nThreads, EventSupplement, LayerSupplement,
+ * Grid&), GridBuilder.h:42.  Counting sort of every (event, layer) segment
+ * into (z, phi) cells; writes the sorted hit columns (+ orig_index, phi) to
+ * `sorted` and the cell starts (n_events * n_layers * (n_z+1)(n_phi+1)
+ * uint32, global sorted indices) to d_grid.  Within a cell hits keep their
+ * input order. */
+int trax_grid_build(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
+                    const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,
+                    const trax_workspace *ws, void *stream);
+
+/* ------------------------------------------------------------------ pairs */
+/* PairGeneratorBeamspot::run(...), PairGeneratorBeamspot.h:40-41, in two
+ * phases.  count: per outer hit pair counts + prefix sum into the workspace;
+ * the total lands in out->totals[0].  store: writes out->pairs and
+ * out->pair_offsets (requires the count phase of the same batch). */
+int trax_pairs_count(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                     const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                     trax_seed_outputs *out, void *stream);
+int trax_pairs_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                     const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                     trax_seed_outputs *out, void *stream);
+
+/* ------------------------------------------------------------- prediction */
+/* TripletThetaPhiPredictor::run(...), TripletThetaPhiPredictor.h:43-44, in
+ * two phases: count (+ prefix sum; total -> out->totals[1]) and store
+ * (out->cands, out->cand_offsets).  The pair count is read on the device. */
+int trax_predict_count(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                       const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                       trax_seed_outputs *out, void *stream);
+int trax_predict_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                       const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                       trax_seed_outputs *out, void *stream);
+
+/* ----------------------------------------------------------------- filter */
+/* TripletThetaPhiFilter::run(...), TripletThetaPhiFilter.h:46-48: cuts and
+ * order-preserving compaction in one pass; writes t1/t2/t3, the offsets and
+ * out->totals[2]. */
+int trax_triplet_filter(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                        const trax_hits *sorted, trax_seed_outputs *out, void *stream);
+
+/* ------------------------------------------------------------ whole chain */
+/* The four stages back to back with no host synchronisation (the stage
+ * sequence of buildTriplets, src/triplet_builder/TripletBuilder.cpp:240-263).
+ * Safe to capture in a hipGraph: no allocation, no memset, no host read. */
+int trax_seed_chain(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
+                    const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,
+                    const trax_workspace *ws, trax_seed_outputs *out, void *stream);
+
+/* ---------------------------------------------------------- test support */
+/* Elementwise device evaluation of the FP32 built-in definitions used by the
+ * kernels: op 0 atan2(a, b), 1 atan(a), 2 acos(a), 3 sign(a), 4 a / b,
+ * 5 sqrt(a).  Lets tests pin the device arithmetic bit for bit. */
+int trax_fp32_eval(trax_ctx *ctx, int op, const float *d_a, const float *d_b, float *d_out,
+                   uint64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRAX_HIP_H */

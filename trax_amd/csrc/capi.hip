@@ -1,0 +1,487 @@
+// C ABI of libtraxhip.so (declarations and reference citations in
+// include/trax_hip.h).  Validates arguments, turns the public configuration
+// into the by-value kernel parameter block, and enqueues the kernels.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/trax_hip.h"
+#include "common.h"
+#include "kernels.h"
+
+static_assert(TRAX_MAX_LAYERS == 16 && TRAX_MAX_TRIPLETS == 8, "table sizes");
+
+struct trax_ctx {
+  int device = 0;
+  std::string err;
+  void* block = nullptr;  // one device allocation: ctrl | flags | status words
+  trax::ScanCtrl* ctrl = nullptr;
+  uint32_t* flags = nullptr;
+  unsigned long long* status = nullptr;
+  size_t lds_max = 65536;
+};
+
+namespace {
+
+constexpr uint64_t kAlign = 256;
+
+uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+int fail(trax_ctx* ctx, int code, const char* msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(trax_ctx* ctx, hipError_t e, const char* where) {
+  if (ctx) {
+    ctx->err = std::string(where) + ": " + hipGetErrorString(e);
+  }
+  return TRAX_ERR_HIP;
+}
+
+#define TRAX_HIP_CHECK(ctx, expr)                          \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+uint32_t bits_for(uint32_t n) {  // bits needed to hold values 0..n-1
+  uint32_t b = 0;
+  while (b < 32 && (1ull << b) < n) ++b;
+  return b;
+}
+
+// Validate the configuration and build the kernel parameter block.
+int make_params(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events, trax::SeedParams* sp) {
+  if (!cfg) return fail(ctx, TRAX_ERR_INVALID, "null configuration");
+  if (cfg->n_z == 0 || cfg->n_phi == 0) return fail(ctx, TRAX_ERR_INVALID, "n_z and n_phi must be > 0");
+  if (cfg->n_layers == 0 || cfg->n_layers > TRAX_MAX_LAYERS)
+    return fail(ctx, TRAX_ERR_INVALID, "n_layers must be in 1..16");
+  if (cfg->n_triplets == 0 || cfg->n_triplets > TRAX_MAX_TRIPLETS)
+    return fail(ctx, TRAX_ERR_INVALID, "n_triplets must be in 1..8");
+  const uint64_t cells = (uint64_t)(cfg->n_z + 1) * (cfg->n_phi + 1);
+  if (cells > (1u << 24)) return fail(ctx, TRAX_ERR_INVALID, "too many grid cells");
+  if ((uint64_t)n_events * cfg->n_layers * cells > 0xFFFFFFFFull)
+    return fail(ctx, TRAX_ERR_INVALID, "grid larger than 2^32 cells");
+  memset(sp, 0, sizeof(*sp));
+  sp->n_events = n_events;
+  sp->n_layers = cfg->n_layers;
+  sp->n_z = cfg->n_z;
+  sp->n_phi = cfg->n_phi;
+  sp->n_triplets = cfg->n_triplets;
+  sp->cells = (uint32_t)cells;
+  sp->cell_bits = bits_for((uint32_t)cells);
+  // GridConfig (Grid.h:9-15, 33-37): float arithmetic on the float constants
+  const float min_z = -300.0f, max_z = 300.0f;
+  const float max_phi = (float)M_PI, min_phi = (float)(-M_PI);
+  sp->min_z = min_z;
+  sp->sector_size_z = (max_z - min_z) / (float)cfg->n_z;
+  sp->min_phi = min_phi;
+  sp->sector_size_phi = (max_phi - min_phi) / (float)cfg->n_phi;
+  sp->min_radius_curvature = cfg->min_radius_curvature;
+  for (uint32_t l = 0; l < TRAX_MAX_LAYERS; ++l) {
+    sp->min_radius[l] = cfg->min_radius[l];
+    sp->max_radius[l] = cfg->max_radius[l];
+  }
+  for (uint32_t p = 0; p < cfg->n_triplets; ++p) {
+    const trax_layer_triplet& t = cfg->triplets[p];
+    // The predictor and filter identify the layer triplet by its innermost
+    // layer (TripletThetaPhiPredictor.h:79, TripletThetaPhiFilter.h:73).
+    if (t.layer1 != p + 1) return fail(ctx, TRAX_ERR_INVALID, "layer triplet p must have layer1 == p + 1");
+    if (t.layer2 < 1 || t.layer2 > cfg->n_layers || t.layer3 < 1 || t.layer3 > cfg->n_layers ||
+        t.layer1 > cfg->n_layers)
+      return fail(ctx, TRAX_ERR_INVALID, "layer triplet names a layer outside 1..n_layers");
+    sp->layer1[p] = t.layer1;
+    sp->layer2[p] = t.layer2;
+    sp->layer3[p] = t.layer3;
+    sp->dtheta_cut[p] = t.dtheta_cut;
+    sp->sigma_z[p] = t.sigma_z;
+    sp->dphi_cut[p] = t.dphi_cut;
+    sp->sigma_phi[p] = t.sigma_phi;
+    sp->tip_cut[p] = t.tip_cut;
+    sp->z0[p] = t.z0;
+    // the pair kernel's sigmaPhi argument is bound to the uint pairSpreadPhi
+    // column (PairGeneratorBeamspot.cpp:55): its bits are read as a float
+    memcpy(&sp->pair_sigma_phi[p], &t.pair_spread_phi, 4);
+  }
+  return TRAX_OK;
+}
+
+trax::SortedHits sorted_view(const trax_hits* h) {
+  trax::SortedHits s;
+  s.x = h->x;
+  s.y = h->y;
+  s.z = h->z;
+  s.phi = h->phi;
+  s.layer = h->layer;
+  s.evt = h->evt;
+  return s;
+}
+
+bool sorted_ok(const trax_hits* h) { return h && h->x && h->y && h->z && h->phi; }
+
+int check_ctx(trax_ctx* ctx) {
+  if (!ctx) return TRAX_ERR_INVALID;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+  return TRAX_OK;
+}
+
+int check_ws(trax_ctx* ctx, const trax_workspace* ws, uint32_t n_events, uint32_t n_triplets) {
+  if (!ws || !ws->work_offsets || !ws->pair_counts || !ws->cand_counts)
+    return fail(ctx, TRAX_ERR_INVALID, "incomplete workspace");
+  if (n_events > ws->max_events) return fail(ctx, TRAX_ERR_INVALID, "more events than the workspace holds");
+  if ((((uintptr_t)ws->pair_counts) | ((uintptr_t)ws->cand_counts)) & 15u)
+    return fail(ctx, TRAX_ERR_INVALID, "workspace counts must be 16-byte aligned");
+  (void)n_triplets;
+  return TRAX_OK;
+}
+
+uint64_t pair_cap(const trax_workspace* ws, const trax_seed_outputs* out) {
+  return out->pair_capacity < ws->pair_capacity ? out->pair_capacity : ws->pair_capacity;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+extern "C" int trax_abi_version(void) { return TRAX_ABI_VERSION; }
+
+extern "C" int trax_ctx_create(int device, trax_ctx** out) {
+  if (!out) return TRAX_ERR_INVALID;
+  *out = nullptr;
+  trax_ctx* ctx = new trax_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return TRAX_ERR_HIP;
+  }
+  const size_t bytes = 256 + trax::kStatusWords * sizeof(unsigned long long);
+  e = hipMalloc(&ctx->block, bytes);
+  if (e != hipSuccess) {
+    delete ctx;
+    return TRAX_ERR_NOMEM;
+  }
+  e = hipMemset(ctx->block, 0, bytes);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(ctx->block);
+    delete ctx;
+    return TRAX_ERR_HIP;
+  }
+  char* b = static_cast<char*>(ctx->block);
+  ctx->ctrl = reinterpret_cast<trax::ScanCtrl*>(b);
+  ctx->flags = reinterpret_cast<uint32_t*>(b + 64);
+  ctx->status = reinterpret_cast<unsigned long long*>(b + 256);
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+      lds > 0)
+    ctx->lds_max = (size_t)lds;
+  // leave room for the block's static LDS (scan scratch)
+  ctx->lds_max = ctx->lds_max > 1024 ? ctx->lds_max - 1024 : 0;
+  if (trax::grid_build_set_lds_limit(ctx->lds_max) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->lds_max = 64512;
+  }
+  *out = ctx;
+  return TRAX_OK;
+}
+
+extern "C" int trax_ctx_destroy(trax_ctx* ctx) {
+  if (!ctx) return TRAX_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->block) (void)hipFree(ctx->block);
+  delete ctx;
+  return TRAX_OK;
+}
+
+extern "C" const char* trax_ctx_last_error(const trax_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int trax_ctx_flags(trax_ctx* ctx, void* stream, uint32_t* flags_out, int clear) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t v = 0;
+  TRAX_HIP_CHECK(ctx, hipMemcpyAsync(&v, ctx->flags, 4, hipMemcpyDeviceToHost, s));
+  TRAX_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  if (clear && v) {
+    const uint32_t zero = 0;
+    TRAX_HIP_CHECK(ctx, hipMemcpyAsync(ctx->flags, &zero, 4, hipMemcpyHostToDevice, s));
+    TRAX_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  }
+  if (flags_out) *flags_out = v;
+  return TRAX_OK;
+}
+
+extern "C" float trax_min_radius_curvature(float min_pt) {
+  // TripletConfiguration.cpp:145-152 with Bz = IDEAL_BZ passed through a float
+  const double Q = 1.602177E-19;
+  const double GEV_C = 5.344286E-19;
+  const float bz = (float)3.8112;
+  return (float)(min_pt * GEV_C / (bz * Q) * 1E2);
+}
+
+extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t max_events,
+                                          uint32_t max_hits, uint64_t pair_capacity, void* base,
+                                          trax_workspace* ws) {
+  if (!cfg) return 0;
+  const uint64_t cells = (uint64_t)(cfg->n_z + 1) * (cfg->n_phi + 1);
+  uint64_t off = 0;
+  const uint64_t o_work = off;
+  off = align_up(off + ((uint64_t)max_events * cfg->n_triplets + 1) * 4);
+  const uint64_t o_pc = off;
+  off = align_up(off + ((uint64_t)max_hits + 1) * 4);
+  const uint64_t o_cc = off;
+  off = align_up(off + (pair_capacity + 1) * 4);
+  uint64_t o_gc = 0;
+  const bool need_gc = cells * 4 > 65536;
+  if (need_gc) {
+    o_gc = off;
+    off = align_up(off + (uint64_t)max_events * cfg->n_layers * cells * 4);
+  }
+  if (ws) {
+    char* b = static_cast<char*>(base);
+    ws->work_offsets = b ? reinterpret_cast<uint32_t*>(b + o_work) : nullptr;
+    ws->pair_counts = b ? reinterpret_cast<uint32_t*>(b + o_pc) : nullptr;
+    ws->cand_counts = b ? reinterpret_cast<uint32_t*>(b + o_cc) : nullptr;
+    ws->grid_counters = (b && need_gc) ? reinterpret_cast<uint32_t*>(b + o_gc) : nullptr;
+    ws->max_events = max_events;
+    ws->max_hits = max_hits;
+    ws->pair_capacity = pair_capacity;
+  }
+  return off;
+}
+
+// --------------------------------------------------------------- prefix sum
+extern "C" int trax_prefix_sum_dev(trax_ctx* ctx, const uint32_t* d_in, uint32_t* d_out,
+                                   uint64_t n_max, const uint32_t* d_n, uint32_t n_add,
+                                   void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (n_max == 0 && !d_n) return TRAX_OK;
+  if (!d_in || !d_out) return fail(ctx, TRAX_ERR_INVALID, "null scan buffer");
+  if ((((uintptr_t)d_in) | ((uintptr_t)d_out)) & 15u)
+    return fail(ctx, TRAX_ERR_INVALID, "scan buffers must be 16-byte aligned");
+  if (n_max > 0xFFFFFFFFull) return fail(ctx, TRAX_ERR_INVALID, "scan longer than 2^32 elements");
+  TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(d_in, d_out, n_max, d_n, n_add, ctx->ctrl, ctx->status,
+                                            static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
+extern "C" int trax_prefix_sum(trax_ctx* ctx, uint32_t* d_data, uint64_t n, void* stream) {
+  if (n == 0) return check_ctx(ctx);
+  return trax_prefix_sum_dev(ctx, d_data, d_data, n, nullptr, 0, stream);
+}
+
+// --------------------------------------------------------------- grid build
+extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const trax_hits* in,
+                               const trax_layout* layout, trax_hits* sorted, uint32_t* d_grid,
+                               const trax_workspace* ws, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!layout) return fail(ctx, TRAX_ERR_INVALID, "null layout");
+  trax::SeedParams sp;
+  rc = make_params(ctx, cfg, layout->n_events, &sp);
+  if (rc) return rc;
+  if (layout->n_events == 0) return TRAX_OK;
+  if (!in || !in->x || !in->y || !in->z || !in->layer || !in->det_id || !in->hit_id || !in->evt)
+    return fail(ctx, TRAX_ERR_INVALID, "incomplete input hits");
+  if (!sorted || !sorted->x || !sorted->y || !sorted->z || !sorted->layer || !sorted->det_id ||
+      !sorted->hit_id || !sorted->evt || !sorted->orig_index || !sorted->phi)
+    return fail(ctx, TRAX_ERR_INVALID, "incomplete sorted-hit buffers");
+  if (!d_grid || !layout->event_offset || !layout->layer_nhits || !layout->layer_offset)
+    return fail(ctx, TRAX_ERR_INVALID, "null grid or layout array");
+  trax::GridLaunch gl;
+  gl.ix = in->x; gl.iy = in->y; gl.iz = in->z;
+  gl.ilayer = in->layer; gl.idet = in->det_id; gl.ihid = in->hit_id; gl.ievt = in->evt;
+  gl.ox = sorted->x; gl.oy = sorted->y; gl.oz = sorted->z;
+  gl.olayer = sorted->layer; gl.odet = sorted->det_id; gl.ohid = sorted->hit_id;
+  gl.oevt = sorted->evt; gl.oorig = sorted->orig_index; gl.ophi = sorted->phi;
+  gl.event_offset = layout->event_offset;
+  gl.layer_nhits = layout->layer_nhits;
+  gl.layer_offset = layout->layer_offset;
+  gl.n_hits = layout->n_hits;
+  gl.grid = d_grid;
+  gl.flags = ctx->flags;
+  gl.counters = nullptr;
+  gl.use_lds = (uint64_t)sp.cells * 4u <= ctx->lds_max;
+  if (!gl.use_lds) {
+    if (!ws || !ws->grid_counters || ws->max_events < layout->n_events)
+      return fail(ctx, TRAX_ERR_INVALID, "grid too large for LDS and no workspace grid counters");
+    gl.counters = ws->grid_counters;
+  }
+  TRAX_HIP_CHECK(ctx, trax::launch_grid_build(sp, gl, static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
+// -------------------------------------------------------------------- pairs
+extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                const trax_hits* sorted, const uint32_t* d_grid,
+                                const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals)
+    return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t K = n_events * cfg->n_triplets;
+  const trax::SortedHits h = sorted_view(sorted);
+  TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
+                                         ws->max_hits, false, ctx->flags, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
+                                            ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s));
+  // pair_offsets[k] = scanned count at the first outer hit of segment k
+  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->pair_counts, ws->max_hits, ws->work_offsets + K,
+                                                  ws->work_offsets, K + 1, out->pair_offsets,
+                                                  out->totals + 0, s));
+  return TRAX_OK;
+}
+
+extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                const trax_hits* sorted, const uint32_t* d_grid,
+                                const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if (!sorted_ok(sorted) || !d_grid || !out || (!out->pairs && out->pair_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
+  TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
+                                         ws->pair_counts, reinterpret_cast<uint2*>(out->pairs),
+                                         out->pair_capacity, ws->max_hits, true, ctx->flags,
+                                         static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
+// ---------------------------------------------------------------- predictor
+extern "C" int trax_predict_count(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                  const trax_hits* sorted, const uint32_t* d_grid,
+                                  const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets ||
+      !out->totals || (!out->pairs && out->pair_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null prediction-stage buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t K = n_events * cfg->n_triplets;
+  const uint64_t pc = pair_cap(ws, out);
+  TRAX_HIP_CHECK(ctx, trax::launch_predict(sp, sorted_view(sorted), d_grid,
+                                           reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
+                                           out->totals + 0, pc, ws->cand_counts, nullptr, 0, false,
+                                           ctx->flags, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->cand_counts, ws->cand_counts, pc + 1, out->totals + 0, 1,
+                                            ctx->ctrl, ctx->status, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->cand_counts, (uint32_t)pc, out->totals + 0,
+                                                  out->pair_offsets, K + 1, out->cand_offsets,
+                                                  out->totals + 1, s));
+  return TRAX_OK;
+}
+
+extern "C" int trax_predict_store(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                  const trax_hits* sorted, const uint32_t* d_grid,
+                                  const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals ||
+      (!out->cands && out->cand_capacity) || (!out->pairs && out->pair_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null prediction-stage buffer");
+  TRAX_HIP_CHECK(ctx, trax::launch_predict(sp, sorted_view(sorted), d_grid,
+                                           reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
+                                           out->totals + 0, pair_cap(ws, out), ws->cand_counts,
+                                           reinterpret_cast<uint2*>(out->cands), out->cand_capacity, true,
+                                           ctx->flags, static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
+// ------------------------------------------------------------------- filter
+extern "C" int trax_triplet_filter(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                   const trax_hits* sorted, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if (!sorted || !sorted->x || !sorted->y || !sorted->z || !out || !out->cand_offsets ||
+      !out->tracklet_offsets || !out->totals || (!out->cands && out->cand_capacity) ||
+      (!out->pairs && out->pair_capacity) ||
+      ((!out->t1 || !out->t2 || !out->t3) && out->tracklet_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null filter-stage buffer");
+  const uint64_t tiles = (out->cand_capacity + 1023) / 1024;
+  if (tiles > trax::kStatusWords) return fail(ctx, TRAX_ERR_INVALID, "candidate capacity too large");
+  TRAX_HIP_CHECK(ctx, trax::launch_filter(sp, sorted_view(sorted), reinterpret_cast<const uint2*>(out->pairs),
+                                          reinterpret_cast<const uint2*>(out->cands), out->cand_offsets,
+                                          out->totals + 1, out->cand_capacity, out->t1, out->t2, out->t3,
+                                          out->tracklet_capacity, out->tracklet_offsets, out->totals + 2,
+                                          ctx->ctrl, ctx->status, ctx->flags,
+                                          static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
+// -------------------------------------------------------------- whole chain
+extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const trax_hits* in,
+                               const trax_layout* layout, trax_hits* sorted, uint32_t* d_grid,
+                               const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = trax_grid_build(ctx, cfg, in, layout, sorted, d_grid, ws, stream);
+  if (rc) return rc;
+  const uint32_t E = layout->n_events;
+  if ((rc = trax_pairs_count(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
+  if ((rc = trax_pairs_store(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
+  if ((rc = trax_predict_count(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
+  if ((rc = trax_predict_store(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
+  return trax_triplet_filter(ctx, cfg, E, sorted, out, stream);
+}
+
+// ------------------------------------------------------------- test support
+namespace trax {
+
+__global__ void k_fp32_eval(int op, const float* __restrict__ a, const float* __restrict__ b,
+                            float* __restrict__ out, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    float r;
+    switch (op) {
+      case 0: r = trax_atan2f(a[i], b[i]); break;
+      case 1: r = trax_atanf(a[i]); break;
+      case 2: r = trax_acosf(a[i]); break;
+      case 3: r = trax_sign(a[i]); break;
+      case 4: r = a[i] / b[i]; break;
+      case 5: r = __builtin_sqrtf(a[i]); break;
+      default: r = 0.0f;
+    }
+    out[i] = r;
+  }
+}
+
+hipError_t launch_fp32_eval(int op, const float* a, const float* b, float* out, uint64_t n,
+                            hipStream_t stream) {
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks == 0) return hipSuccess;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fp32_eval, dim3((uint32_t)blocks), dim3(256), 0, stream, op, a, b, out, n);
+  return hipGetLastError();
+}
+
+}  // namespace trax
+
+extern "C" int trax_fp32_eval(trax_ctx* ctx, int op, const float* d_a, const float* d_b, float* d_out,
+                              uint64_t n, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (op < 0 || op > 5) return fail(ctx, TRAX_ERR_INVALID, "unknown op");
+  if (n && (!d_a || !d_b || !d_out)) return fail(ctx, TRAX_ERR_INVALID, "null buffer");
+  TRAX_HIP_CHECK(ctx, trax::launch_fp32_eval(op, d_a, d_b, d_out, n, static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}

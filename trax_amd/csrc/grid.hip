@@ -1,0 +1,193 @@
+// Grid build: per-(event, layer) (z, phi) counting sort of the hit columns.
+//
+// Reference: src/algorithms/GridBuilder.h:48-427 and GridBuilder.cpp:6-160 run
+// three passes with a host synchronisation and a full grid download between
+// them: gridCount (LDS histogram, global atomics), PrefixSum over the whole
+// E*L*C grid, and gridStore (scatter with atomic_inc ranks, so the order
+// inside a cell is non-deterministic).
+//
+// Here one 256-thread block owns one (event, layer) segment and does all of
+// it in LDS: histogram, in-block exclusive scan of the C cells, grid write,
+// and a *stable* scatter.  The global scan of the reference is not needed:
+// hits are stored event-major and layer-ascending, so the cells of all
+// earlier segments hold exactly the hits before this segment, and the
+// segment's global base is its hit offset.  The stable rank inside a cell is
+// the count of earlier same-cell hits: across 256-hit chunks it comes from
+// running per-cell counters in LDS, inside a wave from a 64-lane match built
+// with one ballot per cell-id bit, and across the four waves of a chunk from
+// taking the waves in order.  The block also writes the input index
+// (`orig_index`) and phi = atan2(y, x) of every sorted hit, which the pair
+// and prediction stages read instead of recomputing atan2.
+#include "common.h"
+#include "kernels.h"
+
+namespace trax {
+
+struct GridArgs {
+  const float* __restrict__ ix;
+  const float* __restrict__ iy;
+  const float* __restrict__ iz;
+  const uint32_t* __restrict__ ilayer;
+  const uint32_t* __restrict__ idet;
+  const uint32_t* __restrict__ ihid;
+  const uint32_t* __restrict__ ievt;
+  float* __restrict__ ox;
+  float* __restrict__ oy;
+  float* __restrict__ oz;
+  uint32_t* __restrict__ olayer;
+  uint32_t* __restrict__ odet;
+  uint32_t* __restrict__ ohid;
+  uint32_t* __restrict__ oevt;
+  uint32_t* __restrict__ oorig;
+  float* __restrict__ ophi;
+  const uint32_t* __restrict__ event_offset;
+  const uint32_t* __restrict__ layer_nhits;
+  const uint32_t* __restrict__ layer_offset;
+  uint32_t n_hits;
+  uint32_t* __restrict__ grid;
+  uint32_t* __restrict__ counters;  // global fallback: E*L*C running counters (else unused)
+  uint32_t* __restrict__ flags;
+};
+
+// In-block exclusive scan of cnt[0..C) (256 threads, 4 consecutive cells per
+// thread per 1024-cell step).  `cnt` is LDS or global memory.
+template <typename Ptr>
+__device__ __forceinline__ void block_exclusive_scan_cells(Ptr cnt, uint32_t C, uint32_t* s_wave,
+                                                           uint32_t* s_carry) {
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < C; base += 1024u) {
+    uint32_t c0 = base + tid * 4u;
+    uint32_t v0 = c0 < C ? cnt[c0] : 0u;
+    uint32_t v1 = c0 + 1 < C ? cnt[c0 + 1] : 0u;
+    uint32_t v2 = c0 + 2 < C ? cnt[c0 + 2] : 0u;
+    uint32_t v3 = c0 + 3 < C ? cnt[c0 + 3] : 0u;
+    uint32_t t = v0 + v1 + v2 + v3;
+    uint32_t incl = wave_inclusive_scan(t);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      uint32_t c = s_wave[w];
+      wp += (w < (int)wave) ? c : 0u;
+      tot += c;
+    }
+    uint32_t e = carry + wp + incl - t;
+    if (c0 < C) cnt[c0] = e;
+    if (c0 + 1 < C) cnt[c0 + 1] = e + v0;
+    if (c0 + 2 < C) cnt[c0 + 2] = e + v0 + v1;
+    if (c0 + 3 < C) cnt[c0 + 3] = e + v0 + v1 + v2;
+    carry += tot;
+    __syncthreads();
+  }
+  (void)s_carry;
+}
+
+template <bool LDS_CELLS>
+__global__ __launch_bounds__(256) void k_grid_build(SeedParams sp, GridArgs a) {
+  extern __shared__ uint32_t s_dyn[];
+  __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_carry;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  const uint32_t C = sp.cells, L = sp.n_layers;
+  const uint32_t nseg = sp.n_events * L;
+  for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    const uint32_t e = seg / L;
+    uint32_t base = a.event_offset[e] + a.layer_offset[seg];
+    uint32_t nh = a.layer_nhits[seg];
+    if (base > a.n_hits || nh > a.n_hits - base) {  // inconsistent layout: flag, emit empty cells
+      if (tid == 0) atomicOr(a.flags, TRAX_FLAG_LAYOUT);
+      base = base > a.n_hits ? a.n_hits : base;
+      nh = 0;
+    }
+    uint32_t* cnt = LDS_CELLS ? s_dyn : (a.counters + (uint64_t)seg * C);
+    for (uint32_t c = tid; c < C; c += 256u) cnt[c] = 0u;
+    __syncthreads();
+    // histogram (gridCount, GridBuilder.h:79-85)
+    for (uint32_t i = tid; i < nh; i += 256u) {
+      float phi;
+      uint32_t cell = grid_cell(sp, a.ix[base + i], a.iy[base + i], a.iz[base + i], &phi);
+      atomicAdd(&cnt[cell], 1u);
+    }
+    __syncthreads();
+    block_exclusive_scan_cells(cnt, C, s_wave, &s_carry);
+    // cell starts in the global sorted array (the reference's scanned grid)
+    uint32_t* g = a.grid + (uint64_t)seg * C;
+    for (uint32_t c = tid; c < C; c += 256u) g[c] = base + cnt[c];
+    __syncthreads();
+    // stable scatter (gridStore, GridBuilder.h:238-253, with a stable rank)
+    for (uint32_t s0 = 0; s0 < nh; s0 += 256u) {
+      const uint32_t i = s0 + tid;
+      const bool valid = i < nh;
+      float x = 0.f, y = 0.f, z = 0.f, phi = 0.f;
+      uint32_t cell = 0;
+      if (valid) {
+        x = a.ix[base + i];
+        y = a.iy[base + i];
+        z = a.iz[base + i];
+        cell = grid_cell(sp, x, y, z, &phi);
+      }
+      uint64_t match = __ballot(valid);
+      for (uint32_t b = 0; b < sp.cell_bits; ++b) {
+        const bool bit = (cell >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        match &= bit ? m : ~m;
+      }
+      const uint32_t rank = lanes_below(match);
+      const uint32_t count = (uint32_t)__popcll(match);
+      uint32_t pos = 0;
+      for (uint32_t w = 0; w < 4u; ++w) {
+        if (wave == w && valid) {
+          pos = cnt[cell] + rank;
+          if (rank == 0) cnt[cell] = pos + count;
+        }
+        __syncthreads();
+      }
+      if (valid) {
+        const uint32_t dst = base + pos;
+        a.ox[dst] = x;
+        a.oy[dst] = y;
+        a.oz[dst] = z;
+        a.olayer[dst] = a.ilayer[base + i];
+        a.odet[dst] = a.idet[base + i];
+        a.ohid[dst] = a.ihid[base + i];
+        a.oevt[dst] = a.ievt[base + i];
+        a.oorig[dst] = base + i;
+        a.ophi[dst] = phi;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream) {
+  GridArgs a;
+  a.ix = gl.ix; a.iy = gl.iy; a.iz = gl.iz;
+  a.ilayer = gl.ilayer; a.idet = gl.idet; a.ihid = gl.ihid; a.ievt = gl.ievt;
+  a.ox = gl.ox; a.oy = gl.oy; a.oz = gl.oz;
+  a.olayer = gl.olayer; a.odet = gl.odet; a.ohid = gl.ohid; a.oevt = gl.oevt;
+  a.oorig = gl.oorig; a.ophi = gl.ophi;
+  a.event_offset = gl.event_offset; a.layer_nhits = gl.layer_nhits; a.layer_offset = gl.layer_offset;
+  a.n_hits = gl.n_hits;
+  a.grid = gl.grid;
+  a.counters = gl.counters;
+  a.flags = gl.flags;
+  const uint32_t nseg = sp.n_events * sp.n_layers;
+  if (nseg == 0) return hipSuccess;
+  const uint32_t blocks = nseg < 65535u ? nseg : 65535u;
+  const size_t lds = (size_t)sp.cells * 4u;
+  if (gl.use_lds) {
+    hipLaunchKernelGGL(k_grid_build<true>, dim3(blocks), dim3(256), lds, stream, sp, a);
+  } else {
+    hipLaunchKernelGGL(k_grid_build<false>, dim3(blocks), dim3(256), 0, stream, sp, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t grid_build_set_lds_limit(size_t bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_grid_build<true>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+}  // namespace trax

@@ -1,0 +1,57 @@
+// Host-side launchers of the gfx950 seeding kernels (internal to libtraxhip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace trax {
+
+// Blocks of a persistent (ticketed) launch: 256 CUs x 8 resident blocks.
+constexpr uint64_t kPersistentBlocks = 2048;
+// Look-back status words reserved per context (tiles of a scan / filter launch).
+constexpr uint64_t kStatusWords = 1ull << 20;
+
+uint64_t scan_tile_elems();
+hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
+                           uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
+                           hipStream_t stream);
+
+struct GridLaunch {
+  const float *ix, *iy, *iz;
+  const uint32_t *ilayer, *idet, *ihid, *ievt;
+  float *ox, *oy, *oz;
+  uint32_t *olayer, *odet, *ohid, *oevt, *oorig;
+  float* ophi;
+  const uint32_t *event_offset, *layer_nhits, *layer_offset;
+  uint32_t n_hits;
+  uint32_t* grid;
+  uint32_t* counters;
+  uint32_t* flags;
+  bool use_lds;
+};
+hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream);
+hipError_t grid_build_set_lds_limit(size_t bytes);
+
+hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t* work_off,
+                            hipStream_t stream);
+hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                        const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
+                        uint32_t max_work, bool store, uint32_t* flags, hipStream_t stream);
+hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
+                                 const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
+                                 hipStream_t stream);
+hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                          const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
+                          uint64_t pair_capacity, uint32_t* counts, uint2* cands,
+                          uint64_t cand_capacity, bool store, uint32_t* flags, hipStream_t stream);
+hipError_t launch_filter(const SeedParams& sp, const SortedHits& h, const uint2* pairs,
+                         const uint2* cands, const uint32_t* cand_off, const uint32_t* d_ncands,
+                         uint64_t cand_capacity, uint32_t* t1, uint32_t* t2, uint32_t* t3,
+                         uint64_t trk_capacity, uint32_t* trk_off, uint32_t* d_ntrk, ScanCtrl* ctrl,
+                         unsigned long long* status, uint32_t* flags, hipStream_t stream);
+hipError_t launch_fp32_eval(int op, const float* a, const float* b, float* out, uint64_t n,
+                            hipStream_t stream);
+
+}  // namespace trax

@@ -1,0 +1,228 @@
+// Hit-pair generation with a beam-spot constrained window.
+//
+// Reference: src/algorithms/PairGeneratorBeamspot.h:43-525 and
+// PairGeneratorBeamspot.cpp:5-177.  There, one work-group per (event, layer
+// pair) strides over the outer-layer hits, walks the inner-layer grid cells
+// of each hit's (z, phi) window, and records valid pairs in an n1*n2 bitmap
+// plus a per-work-item count; after a scan, pairStore re-reads all n1 bits of
+// every outer hit to write the pairs.
+//
+// Here the outer hits of all (event, layer pair) segments form one flat work
+// list (offsets from k_pair_plan), so a launch covers every event of a batch
+// and uneven events cost nothing extra.  Count and store run the same window
+// walk (no bitmap): count writes one uint32 per outer hit, the exclusive scan
+// turns them into write positions, store re-walks and writes (inner, outer)
+// pairs.  The window arithmetic is the reference's, expression for
+// expression; the inner-hit phi comes from the phi column written by the grid
+// build (the same atan2 of the same inputs).  The walk visits each inner index
+// once: the bitmap of the reference also collapses the rare repeated visits
+// of its wrap-around walk, so the emitted pair set is the same.
+#include "common.h"
+#include "kernels.h"
+
+namespace trax {
+
+struct PairWindow {
+  float zLow, zHigh, phiLow, phiHigh;
+  bool wrap;
+  uint32_t zLowSector, zHighSector, phiLowSector, phiHighSector;
+};
+
+// PairGeneratorBeamspot.h:110-162 for outer hit (X, Y, Z) with phi = atan2(Y, X).
+__device__ __forceinline__ PairWindow pair_window(const SeedParams& sp, uint32_t p, float X, float Y,
+                                                  float Z, float phi, float minR1, float maxR1) {
+  const float z0 = sp.z0[p], tip = sp.tip_cut[p], sigmaZ = sp.sigma_z[p];
+  const float sigmaPhi = sp.pair_sigma_phi[p];
+  const float R = sp.min_radius_curvature;
+  const float PI = TRAX_PI_F;
+  PairWindow w;
+  float signRadius = trax_sign(Y);
+  float r = signRadius * __builtin_sqrtf(X * X + Y * Y);
+  float cotThetaHigh = (Z - z0) / r;
+  float cotThetaLow = (Z + z0) / r;
+  float tmp = signRadius * maxR1 * cotThetaHigh + z0;
+  float zHigh = signRadius * minR1 * cotThetaHigh + z0;
+  zHigh = trax_b2f(tmp < zHigh) * zHigh + trax_b2f(tmp > zHigh) * tmp;
+  tmp = signRadius * maxR1 * cotThetaLow - z0;
+  float zLow = signRadius * minR1 * cotThetaLow - z0;
+  zLow = trax_b2f(tmp < zLow) * tmp + trax_b2f(tmp > zLow) * zLow;
+  zLow -= sigmaZ;
+  zHigh += sigmaZ;
+  int32_t zl = trax_f2i(__builtin_floorf((zLow - sp.min_z) / sp.sector_size_z));
+  w.zLowSector = (uint32_t)(zl > 0 ? zl : 0);
+  uint32_t zh = trax_f2u(__builtin_floorf((zHigh - sp.min_z) / sp.sector_size_z)) + 1u;
+  w.zHighSector = zh < sp.n_z ? zh : sp.n_z;
+
+  tmp = __builtin_fabsf(trax_acosf(r / (2 * R)) - trax_acosf(maxR1 / (2 * R)));
+  float dPhi = __builtin_fabsf(trax_acosf(r / (2 * R)) - trax_acosf(minR1 / (2 * R)));
+  dPhi = trax_b2f(tmp < dPhi) * dPhi + trax_b2f(tmp > dPhi) * tmp;
+  tmp = trax_atanf(tip * (r - minR1)) / (r * minR1);
+  float phiHigh = trax_atanf(tip * (r - maxR1)) / (r * maxR1);
+  dPhi += tmp > phiHigh ? tmp : phiHigh;
+  float phiLow = phi - dPhi - sigmaPhi;
+  phiHigh = phi + dPhi + sigmaPhi;
+  w.wrap = phiLow < -PI || phiHigh > PI || phiLow > PI || phiHigh < -PI;
+  phiLow -= (float)((phiLow > PI) * 2) * PI;
+  phiLow += (float)((phiLow < -PI) * 2) * PI;
+  phiHigh -= (float)((phiHigh > PI) * 2) * PI;
+  phiHigh += (float)((phiHigh < -PI) * 2) * PI;
+  uint32_t pl = trax_f2u(__builtin_floorf((phiLow - sp.min_phi) / sp.sector_size_phi));
+  uint32_t ph = trax_f2u(__builtin_floorf((phiHigh - sp.min_phi) / sp.sector_size_phi)) + 1u;
+  w.phiLowSector = pl > sp.n_phi ? sp.n_phi : pl;
+  w.phiHighSector = ph > sp.n_phi + 1 ? sp.n_phi + 1 : ph;
+  w.zLow = zLow;
+  w.zHigh = zHigh;
+  w.phiLow = phiLow;
+  w.phiHigh = phiHigh;
+  return w;
+}
+
+__device__ __forceinline__ bool window_phi_ok(bool wrap, float phiLow, float phiHigh, float actPhi) {
+  const float PI = TRAX_PI_F;
+  return (!wrap && (phiLow <= actPhi && actPhi <= phiHigh)) ||
+         (wrap && ((phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= phiHigh)));
+}
+
+// Per-(event, layer pair) work offsets: prefix sum of outer-layer hit counts.
+// nHits2 = grid[C-1] - grid[0] of the outer layer (PairGeneratorBeamspot.h:70-73).
+__global__ __launch_bounds__(256) void k_pair_plan(SeedParams sp, const uint32_t* __restrict__ grid,
+                                                   uint32_t* __restrict__ work_off) {
+  __shared__ uint32_t s_wave[4];
+  const uint32_t K = sp.n_events * sp.n_triplets, C = sp.cells, L = sp.n_layers;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < K; base += 256u) {
+    const uint32_t k = base + tid;
+    uint32_t v = 0;
+    if (k < K) {
+      const uint32_t e = k / sp.n_triplets, p = k % sp.n_triplets;
+      const uint32_t* g2 = grid + ((uint64_t)e * L + (sp.layer2[p] - 1)) * C;
+      v = g2[C - 1] - g2[0];
+    }
+    uint32_t incl = wave_inclusive_scan(v);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+    for (int w = 0; w < 4; ++w) {
+      wp += (w < (int)wave) ? s_wave[w] : 0u;
+      tot += s_wave[w];
+    }
+    if (k < K) work_off[k] = carry + wp + incl - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (tid == 0) work_off[K] = carry;
+}
+
+template <bool STORE>
+__global__ __launch_bounds__(256) void k_pairs(SeedParams sp, SortedHits h,
+                                               const uint32_t* __restrict__ grid,
+                                               const uint32_t* __restrict__ work_off,
+                                               uint32_t* __restrict__ counts,  // count: out; store: scanned
+                                               uint2* __restrict__ pairs, uint64_t capacity,
+                                               uint32_t* __restrict__ flags) {
+  const uint32_t P = sp.n_triplets, K = sp.n_events * P, C = sp.cells, L = sp.n_layers;
+  const uint32_t nPhi = sp.n_phi;
+  const uint32_t W = work_off[K];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += stride) {
+    const uint32_t k = segment_of(work_off, K + 1, w);
+    const uint32_t e = k / P, p = k - e * P;
+    const uint32_t* g2 = grid + ((uint64_t)e * L + (sp.layer2[p] - 1)) * C;
+    const uint32_t l1 = sp.layer1[p] - 1;
+    const uint32_t* g1 = grid + ((uint64_t)e * L + l1) * C;
+    const uint32_t i = g2[0] + (w - work_off[k]);
+    const PairWindow win =
+        pair_window(sp, p, h.x[i], h.y[i], h.z[i], h.phi[i], sp.min_radius[l1], sp.max_radius[l1]);
+    uint32_t n = 0;
+    uint64_t pos = STORE ? counts[w] : 0;
+    bool overflow = false;
+    auto visit = [&](uint32_t a, uint32_t b) {
+      for (uint32_t idx = a; idx < b; ++idx) {
+        const float zi = h.z[idx];
+        bool valid = win.zLow <= zi && zi <= win.zHigh;
+        valid = valid && window_phi_ok(win.wrap, win.phiLow, win.phiHigh, h.phi[idx]);
+        if (valid) {
+          if (STORE) {
+            if (pos < capacity)
+              pairs[pos] = make_uint2(idx, i);
+            else
+              overflow = true;
+            ++pos;
+          } else {
+            ++n;
+          }
+        }
+      }
+    };
+    for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
+      const uint32_t row = zs * (nPhi + 1);
+      const uint32_t zSectorStart = g1[row];
+      const uint32_t zSectorEnd = g1[row + nPhi];  // (zs+1)(nPhi+1)-1: padding cell excluded
+      const uint32_t len = zSectorEnd - zSectorStart;
+      const uint32_t A = g1[row + win.phiLowSector];
+      const uint32_t H = g1[row + win.phiHighSector];
+      const uint32_t E = win.wrap ? zSectorEnd + H - zSectorStart : H;
+      // indices j in [A, E) map to j - (j >= zSectorEnd) * len; visit each once
+      visit(A, E < zSectorEnd ? E : zSectorEnd);
+      if (E > zSectorEnd) {
+        const uint32_t m = E - len;
+        visit(zSectorStart, m < A ? m : A);
+        if (m > zSectorEnd) visit(zSectorEnd, m);
+      }
+    }
+    if (!STORE) counts[w] = n;
+    if (STORE && overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
+  }
+}
+
+// out[k] = scanned[min(idx[k], last)] for k in [0, n); total -> *d_total.
+__global__ __launch_bounds__(256) void k_gather_offsets(const uint32_t* __restrict__ scanned,
+                                                        uint32_t last,
+                                                        const uint32_t* __restrict__ last_dev,
+                                                        const uint32_t* __restrict__ idx, uint32_t n,
+                                                        uint32_t* __restrict__ out,
+                                                        uint32_t* __restrict__ d_total) {
+  uint32_t lim = last;
+  if (last_dev) lim = *last_dev < last ? *last_dev : last;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    uint32_t j = idx[k];
+    j = j < lim ? j : lim;
+    uint32_t v = scanned[j];
+    out[k] = v;
+    if (k == n - 1 && d_total) *d_total = v;
+  }
+}
+
+hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t* work_off,
+                            hipStream_t stream) {
+  hipLaunchKernelGGL(k_pair_plan, dim3(1), dim3(256), 0, stream, sp, grid, work_off);
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                        const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
+                        uint32_t max_work, bool store, uint32_t* flags, hipStream_t stream) {
+  uint32_t blocks = (max_work + 255u) / 256u;
+  if (blocks == 0) blocks = 1;
+  if (blocks > 8192u) blocks = 8192u;
+  if (store)
+    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, stream, sp, h, grid, work_off,
+                       counts, pairs, capacity, flags);
+  else
+    hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, stream, sp, h, grid, work_off,
+                       counts, pairs, capacity, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
+                                 const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
+                                 hipStream_t stream) {
+  uint32_t blocks = (n + 255u) / 256u;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_gather_offsets, dim3(blocks), dim3(256), 0, stream, scanned, last, last_dev,
+                     idx, n, out, d_total);
+  return hipGetLastError();
+}
+
+}  // namespace trax

@@ -1,0 +1,133 @@
+// Exclusive uint32 prefix sum: single pass, decoupled look-back.
+//
+// Replaces the reference's three-kernel recursive Blelloch scan
+// (src/algorithms/PrefixSum.h:55-245, PrefixSum.cpp:60-126), which reads and
+// writes every element about twice per level.  Here each 256-thread block
+// owns one tile of SCAN_TILE elements, loaded and stored as coalesced uint4,
+// reduces it with wave64 shuffles, publishes the tile aggregate, and resolves
+// its exclusive prefix by looking back over the status words of its
+// predecessors (one wave, 64 predecessors per probe).  Traffic is one read
+// and one write per element (8 B).  The result is the exclusive scan mod 2^32,
+// identical to the reference's for every n (the reference's float launch
+// sizing, PrefixSum.cpp:37,66, can drop tail elements above 2^24; this one
+// does not).
+#include "common.h"
+#include "kernels.h"
+
+namespace trax {
+
+template <int SUB>
+__global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
+                                                  uint32_t* out, uint64_t n_max,
+                                                  const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                  ScanCtrl* ctrl, unsigned long long* status) {
+  constexpr uint32_t TILE = 256u * 4u * SUB;
+  __shared__ uint32_t s_tile, s_epoch, s_excl;
+  __shared__ uint32_t s_wave[SUB][4];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  if (tid == 0) s_epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, TRAX_AGENT) & 0x3FFFFFFFu;
+  uint64_t n = n_max;
+  if (d_n) {
+    const uint64_t dn = (uint64_t)d_n[0] + n_add;
+    n = dn < n_max ? dn : n_max;
+  }
+  const uint64_t ntiles = (n + TILE - 1) / TILE;
+  // Persistent: each block takes tiles in ticket order until none are left.
+  while (true) {
+    if (tid == 0) s_tile = atomicAdd(&ctrl->ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile, epoch = s_epoch;
+    if (tile >= ntiles) break;
+    const uint64_t base = (uint64_t)tile * TILE;
+    uint4 v[SUB];
+    uint32_t t[SUB], incl[SUB];
+#pragma unroll
+    for (int s = 0; s < SUB; ++s) {
+      const uint64_t idx = base + (uint64_t)s * 1024u + tid * 4u;
+      if (idx + 4 <= n) {
+        v[s] = *reinterpret_cast<const uint4*>(in + idx);
+      } else {
+        v[s].x = idx < n ? in[idx] : 0u;
+        v[s].y = idx + 1 < n ? in[idx + 1] : 0u;
+        v[s].z = idx + 2 < n ? in[idx + 2] : 0u;
+        v[s].w = idx + 3 < n ? in[idx + 3] : 0u;
+      }
+      t[s] = v[s].x + v[s].y + v[s].z + v[s].w;
+      incl[s] = wave_inclusive_scan(t[s]);
+      if (lane == 63) s_wave[s][wave] = incl[s];
+    }
+    __syncthreads();
+    uint32_t running = 0, thread_excl[SUB];
+#pragma unroll
+    for (int s = 0; s < SUB; ++s) {
+      uint32_t wp = 0, tot = 0;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const uint32_t c = s_wave[s][w];
+        wp += (w < (int)wave) ? c : 0u;
+        tot += c;
+      }
+      thread_excl[s] = running + wp + incl[s] - t[s];
+      running += tot;
+    }
+    const uint32_t aggregate = running;
+    if (wave == 0) {
+      uint32_t excl = 0;
+      if (tile == 0) {
+        if (lane == 0) lookback_publish(status, tile, epoch, 2u, aggregate);
+      } else {
+        if (lane == 0) lookback_publish(status, tile, epoch, 1u, aggregate);
+        excl = lookback_wave(status, tile, epoch);
+        if (lane == 0) lookback_publish(status, tile, epoch, 2u, excl + aggregate);
+      }
+      if (lane == 0) s_excl = excl;
+    }
+    __syncthreads();
+    const uint32_t excl_tile = s_excl;
+#pragma unroll
+    for (int s = 0; s < SUB; ++s) {
+      const uint64_t idx = base + (uint64_t)s * 1024u + tid * 4u;
+      uint4 o;
+      o.x = excl_tile + thread_excl[s];
+      o.y = o.x + v[s].x;
+      o.z = o.y + v[s].y;
+      o.w = o.z + v[s].z;
+      if (idx + 4 <= n) {
+        *reinterpret_cast<uint4*>(out + idx) = o;
+      } else {
+        if (idx < n) out[idx] = o.x;
+        if (idx + 1 < n) out[idx + 1] = o.y;
+        if (idx + 2 < n) out[idx + 2] = o.z;
+        if (idx + 3 < n) out[idx + 3] = o.w;
+      }
+    }
+    __syncthreads();  // s_tile, s_wave and s_excl are reused by the next tile
+  }
+  if (tid == 0) {
+    const uint32_t epoch = s_epoch;
+    const uint32_t d = __hip_atomic_fetch_add(&ctrl->done, 1u, __ATOMIC_ACQ_REL, TRAX_AGENT);
+    if (d == gridDim.x - 1) {  // last block of the launch: reset for the next one
+      __hip_atomic_store(&ctrl->done, 0u, __ATOMIC_RELAXED, TRAX_AGENT);
+      __hip_atomic_store(&ctrl->ticket, 0u, __ATOMIC_RELAXED, TRAX_AGENT);
+      __hip_atomic_store(&ctrl->epoch, (epoch + 1u) & 0x3FFFFFFFu, __ATOMIC_RELEASE, TRAX_AGENT);
+    }
+  }
+}
+
+static constexpr int kScanSub = 8;
+static constexpr uint64_t kScanTile = 256ull * 4ull * kScanSub;
+
+uint64_t scan_tile_elems() { return kScanTile; }
+
+hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
+                           uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
+                           hipStream_t stream) {
+  uint64_t blocks = (n_max + kScanTile - 1) / kScanTile;
+  if (blocks == 0) blocks = 1;
+  if (blocks > kPersistentBlocks) blocks = kPersistentBlocks;
+  hipLaunchKernelGGL(k_scan_u32<kScanSub>, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out,
+                     n_max, d_n, n_add, ctrl, status);
+  return hipGetLastError();
+}
+
+}  // namespace trax
