@@ -1,0 +1,235 @@
+"""Benchmark of the track-seeding chain on one MI355X.
+
+One step = the whole chain (grid build -> pairs -> triplet prediction ->
+triplet filter, with the prefix sums between them) over one batch of
+synthetic events already resident in HBM, replayed from a captured hipGraph.
+The default workload is the high-pile-up batch: 64 events at pile-up 200 on
+six barrel layers with four layer triplets.
+
+Printed as one JSON line: batched events/s (value), triplets/s, the plain
+one-event-per-launch path on the same events, the roofline of the dominant
+kernel (HBM bytes it must move / its measured duration) and the CPU oracle
+baseline on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=3, help="workload configuration 0..3")
+    ap.add_argument("--events", type=int, default=None)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-graph", action="store_true")
+    return ap.parse_args()
+
+
+def time_events(fn, reps: int, torch):
+    """Average device duration of fn() on the current stream (HIP events)."""
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def cpu_baseline(sc, batch, threads: int):
+    """The oracle chain over every event of the batch, a thread per event
+    (ctypes releases the GIL), timed on the host cores."""
+    import concurrent.futures as cf
+
+    from oracle import bridge
+    from trax_amd import events as ev
+
+    singles = [ev.concat_events([batch], subset=[e]) for e in range(batch.n_events)]
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(lambda b: len(bridge.run_chain(sc, b)["tracklets"]), singles))
+    dt = time.perf_counter() - t0
+    return batch.n_events / dt, sum(res), dt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
+                                   "in DESIGN.md but not built", "n_gpus": args.gpus}))
+        return 2
+    import numpy as np
+    import torch
+
+    from trax_amd import events as ev
+    from trax_amd import runtime
+
+    torch.cuda.set_device(0)
+    w = ev.config_workload(args.config, args.events)
+    sc = w.seed_config
+    log(f"generating {w.n_events} events ({w.name})")
+    batch = ev.make_batch(w)
+    log(f"{batch.n_hits} hits")
+    ctx = runtime.Context(0)
+    bufs, db = runtime.run_batch(ctx, sc, batch)
+    torch.cuda.synchronize()
+    n_pairs, n_cands, n_trk = runtime.totals(bufs)
+    log(f"pairs {n_pairs} candidates {n_cands} triplets {n_trk}")
+    ref = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+
+    # ---- batched chain, captured once as a graph and replayed
+    if args.no_graph:
+        def step():
+            runtime.seed_chain(ctx, bufs, db)
+    else:
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            runtime.seed_chain(ctx, bufs, db)  # eager pass on the capture stream
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph):
+            runtime.seed_chain(ctx, bufs, db)
+        torch.cuda.synchronize()
+
+        def step():
+            graph.replay()
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(args.steps):
+        step()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    flags = ctx.flags()
+    got = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+    same = all(np.array_equal(got[k], ref[k]) for k in ("grid", "pairs", "cands", "tracklets",
+                                                         "tracklet_offsets"))
+    if flags or not same:
+        raise SystemExit(f"replayed chain differs from the eager run (flags {flags}, same {same})")
+    ev_s = batch.n_events / (ms / 1e3)
+    log(f"batched: {ms:.3f} ms/step, {ev_s:.0f} events/s (wall {wall:.3f} s)")
+
+    # ---- per-stage device time (eager, HIP events on the launch stream)
+    st = {}
+    st["grid_build"] = time_events(lambda: runtime.grid_build(ctx, bufs, db), 5, torch)
+    st["pairs_count"] = time_events(lambda: runtime.pairs_count(ctx, bufs, db.n_events), 5, torch)
+    st["pairs_store"] = time_events(lambda: runtime.pairs_store(ctx, bufs, db.n_events), 5, torch)
+    st["predict_count"] = time_events(lambda: runtime.predict_count(ctx, bufs, db.n_events), 5, torch)
+    st["predict_store"] = time_events(lambda: runtime.predict_store(ctx, bufs, db.n_events), 5, torch)
+    st["filter"] = time_events(lambda: runtime.triplet_filter(ctx, bufs, db.n_events), 5, torch)
+    log("stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st.items()))
+
+    # grid build roofline: 7 input columns read, 9 sorted columns written (x, y,
+    # z, layer, detId, hitId, evt, orig_index, phi), one uint32 per cell written
+    C = sc.cells_per_layer
+    grid_bytes = batch.n_hits * (7 * 4 + 9 * 4) + batch.n_events * sc.n_layers * C * 4
+    grid_gbs = grid_bytes / (st["grid_build"] * 1e-3) / 1e9
+    # scan roofline on a large array (4 B read + 4 B write per element)
+    n_scan = 1 << 26
+    big = torch.zeros(n_scan, dtype=torch.int32, device="cuda")
+    scan_ms = time_events(lambda: runtime.prefix_sum(ctx, big), 10, torch)
+    scan_gbs = 8 * n_scan / (scan_ms * 1e-3) / 1e9
+    log(f"grid build {grid_gbs:.0f} GB/s, scan(2^26) {scan_gbs:.0f} GB/s")
+    dominant = max(st, key=st.get)
+
+    # ---- plain path: one event per launch sequence, host reads between stages
+    plain = None
+    if not args.no_plain:
+        singles = [runtime.DeviceBatch.upload(ev.concat_events([batch], subset=[e]))
+                   for e in range(batch.n_events)]
+        pb = runtime.ChainBuffers(sc, 1, max(d.n_hits for d in singles), bufs.pair_capacity,
+                                  bufs.cand_capacity, bufs.tracklet_capacity)
+        runtime.run_plain_event(ctx, pb, singles[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n_plain = 0
+        for d in singles:
+            n_plain += runtime.run_plain_event(ctx, pb, d)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        plain = {"events_per_sec": len(singles) / dt, "triplets": n_plain,
+                 "ms_per_event": dt * 1e3 / len(singles)}
+        if n_plain != n_trk:
+            raise SystemExit(f"plain path found {n_plain} triplets, batched {n_trk}")
+        log(f"plain: {plain['ms_per_event']:.3f} ms/event, {plain['events_per_sec']:.0f} events/s")
+
+    # ---- CPU baseline (oracle, bounded sample = this batch)
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        log(f"cpu baseline on {threads} threads")
+        cps, ctrip, cdt = cpu_baseline(sc, batch, threads)
+        if ctrip != n_trk:
+            raise SystemExit(f"CPU oracle found {ctrip} triplets, GPU {n_trk}")
+        cpu = {"value": cps, "unit": "events/s", "cores": threads, "kind": "port",
+               "sample": f"{batch.n_events} events of the same workload, C oracle chain, "
+                         f"{threads} threads, {cdt:.1f} s wall"}
+
+    dom_ms = st[dominant]
+    out = {
+        "metric": "events/sec (batched seeding chain, 1 GPU)",
+        "value": ev_s,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+u32",
+        "data": "synthetic (seeded helix generator)",
+        "config": {"workload": w.name, "events": batch.n_events, "hits": batch.n_hits,
+                   "layers": sc.n_layers, "layer_triplets": len(sc.triplets),
+                   "grid": f"{sc.n_phi}phi x {sc.n_z}z", "graph": not args.no_graph},
+        "triplets_per_sec": n_trk / (ms / 1e3),
+        "counts": {"pairs": n_pairs, "candidates": n_cands, "triplets": n_trk},
+        "plain": plain,
+        "batched_over_plain": (ev_s / plain["events_per_sec"]) if plain else None,
+        "stages_ms": st,
+        "grid_build_roofline": {"achieved": grid_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": grid_gbs / HBM_PEAK_GBS},
+        "scan_roofline": {"n": n_scan, "achieved": scan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": scan_gbs / HBM_PEAK_GBS, "ms": scan_ms},
+        "roofline": {"kernel": dominant, "bound": "hbm", "ms": dom_ms, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "achieved": None, "frac": None, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

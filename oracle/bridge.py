@@ -86,6 +86,11 @@ def lib():
         L.oracle_eval_f32.argtypes = [C.c_int, _f32p, _f32p, _f32p, C.c_uint64]
         L.oracle_cell_of.argtypes = [C.POINTER(OracleCfg), C.c_float, C.c_float, C.c_float, _f32p]
         L.oracle_cell_of.restype = C.c_uint32
+        L.oracle_pair_window.argtypes = [C.POINTER(OracleCfg), C.c_uint32, C.c_float, C.c_float,
+                                         C.c_float, _f32p]
+        L.oracle_predict_window.argtypes = [C.POINTER(OracleCfg), C.c_uint32, C.c_int, _f32p, _f32p,
+                                            C.c_float, C.c_float, _f32p]
+        L.oracle_filter_eval.argtypes = [C.POINTER(OracleCfg), C.c_uint32, _f32p, _f32p, _f32p, _f32p]
         _lib = L
     return _lib
 
@@ -187,3 +192,46 @@ def eval_f32(op: str, a: np.ndarray, b: np.ndarray = None) -> np.ndarray:
     out = np.empty_like(a)
     lib().oracle_eval_f32(OPS[op], _p(a, _f32p), _p(b, _f32p), _p(out, _f32p), a.shape[0])
     return out
+
+
+WINDOW_FIELDS = ("zLow", "zHigh", "phiLow", "phiHigh", "wrap", "zLowSector", "zHighSector",
+                 "phiLowSector", "phiHighSector")
+
+
+def _window(vals):
+    d = dict(zip(WINDOW_FIELDS, vals))
+    for k in WINDOW_FIELDS[4:]:
+        d[k] = int(d[k])
+    return d
+
+
+def pair_window(sc, p: int, x: float, y: float, z: float) -> dict:
+    cfg = make_cfg(sc, 1)
+    out = (C.c_float * 9)()
+    lib().oracle_pair_window(C.byref(cfg), p, x, y, z, out)
+    return _window(list(out))
+
+
+def predict_window(sc, lt: int, store: bool, p1, p2, min_r: float, max_r: float) -> dict:
+    cfg = make_cfg(sc, 1)
+    a = (C.c_float * 3)(*[float(v) for v in p1])
+    b = (C.c_float * 3)(*[float(v) for v in p2])
+    out = (C.c_float * 9)()
+    lib().oracle_predict_window(C.byref(cfg), lt, int(store), a, b, min_r, max_r, out)
+    return _window(list(out))
+
+
+def filter_eval(sc, lt: int, h1, h2, h3) -> dict:
+    cfg = make_cfg(sc, 1)
+    arrs = [(C.c_float * 3)(*[float(v) for v in h]) for h in (h1, h2, h3)]
+    out = (C.c_float * 5)()
+    lib().oracle_filter_eval(C.byref(cfg), lt, arrs[0], arrs[1], arrs[2], out)
+    return {"theta_ratio": out[0], "dphi": out[1], "tip2": out[2], "radius": out[3],
+            "pass": bool(out[4])}
+
+
+def cell_of(sc, x: float, y: float, z: float):
+    cfg = make_cfg(sc, 1)
+    phi = C.c_float()
+    cell = lib().oracle_cell_of(C.byref(cfg), x, y, z, C.byref(phi))
+    return int(cell), phi.value

@@ -431,8 +431,8 @@ static inline f3 f3_normalize(f3 v) {
 }
 
 // filterCount predicate, TripletThetaPhiFilter.h:68-147.
-static int filter_pass(const oracle_cfg *c, uint32_t lt, float x1, float y1, float z1, float x2,
-                       float y2, float z2, float x3, float y3, float z3) {
+static int filter_pass_ex(const oracle_cfg *c, uint32_t lt, float x1, float y1, float z1, float x2,
+                          float y2, float z2, float x3, float y3, float z3, float *info) {
   const float dThetaCut = c->dtheta_cut[lt], dPhiCut = c->dphi_cut[lt], tipCut = c->tip_cut[lt];
   const float minRadius = c->min_radius_curvature;
   int valid = 1;
@@ -440,12 +440,14 @@ static int filter_pass(const oracle_cfg *c, uint32_t lt, float x1, float y1, flo
   float angle2 = o_atan2f(sqrtf((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2)), (z3 - z2));
   float delta = fabsf(angle2 / angle1);
   valid = valid && (1 - dThetaCut <= delta && delta <= 1 + dThetaCut);
+  if (info) info[0] = delta;
 
   angle1 = o_atan2f((y2 - y1), (x2 - x1));
   angle2 = o_atan2f((y3 - y2), (x3 - x2));
   delta = angle2 - angle1;
   delta += (delta > PI) ? -2 * PI : (delta < -PI) ? 2 * PI : 0;
   valid = valid && (fabsf(delta) <= dPhiCut);
+  if (info) info[1] = fabsf(delta);
 
   f3 pP1 = {x1, y1, x1 * x1 + y1 * y1};
   f3 pP2 = {x2, y2, x2 * x2 + y2 * y2};
@@ -469,7 +471,17 @@ static int filter_pass(const oracle_cfg *c, uint32_t lt, float x1, float y1, flo
   float tip = pCAx * pCAx + pCAy * pCAy;
   valid = valid && (tip <= (tipCut * tipCut));
   valid = valid && (cR >= minRadius);
+  if (info) {
+    info[2] = tip;
+    info[3] = cR;
+    info[4] = (float)valid;
+  }
   return valid;
+}
+
+static int filter_pass(const oracle_cfg *c, uint32_t lt, float x1, float y1, float z1, float x2,
+                       float y2, float z2, float x3, float y3, float z3) {
+  return filter_pass_ex(c, lt, x1, y1, z1, x2, y2, z2, x3, y3, z3, 0);
 }
 
 static int filter_stage(const oracle_cfg *c, oracle_result *r) {
@@ -574,4 +586,38 @@ void oracle_eval_f32(int op, const float *a, const float *b, float *out, uint64_
       default: out[i] = 0.0f;
     }
   }
+}
+
+void oracle_pair_window(const oracle_cfg *c, uint32_t p, float x, float y, float z, float *out) {
+  window_t w;
+  pair_window(c, p, x, y, z, c->min_radius[c->layer1[p] - 1], c->max_radius[c->layer1[p] - 1], &w);
+  out[0] = w.zLow;
+  out[1] = w.zHigh;
+  out[2] = w.phiLow;
+  out[3] = w.phiHigh;
+  out[4] = (float)w.wrap;
+  out[5] = (float)w.zLowSector;
+  out[6] = (float)w.zHighSector;
+  out[7] = (float)w.phiLowSector;
+  out[8] = (float)w.phiHighSector;
+}
+
+void oracle_predict_window(const oracle_cfg *c, uint32_t lt, int store, const float *p1,
+                           const float *p2, float minR, float maxR, float *out) {
+  pwindow_t w;
+  predict_window(c, lt, store, p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], minR, maxR, &w);
+  out[0] = w.zLow;
+  out[1] = w.zHigh;
+  out[2] = w.phiLow;
+  out[3] = w.phiHigh;
+  out[4] = (float)w.wrap;
+  out[5] = (float)w.zLowSector;
+  out[6] = (float)w.zHighSector;
+  out[7] = (float)w.phiLowSector;
+  out[8] = (float)w.phiHighSector;
+}
+
+void oracle_filter_eval(const oracle_cfg *c, uint32_t lt, const float *h1, const float *h2,
+                        const float *h3, float *out) {
+  filter_pass_ex(c, lt, h1[0], h1[1], h1[2], h2[0], h2[1], h2[2], h3[0], h3[1], h3[2], out);
 }

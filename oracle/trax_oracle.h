@@ -98,6 +98,16 @@ void oracle_eval_f32(int op, const float *a, const float *b, float *out, uint64_
 // Grid cell of one hit (GridBuilder.h:80-84); phi_out receives atan2(y, x).
 uint32_t oracle_cell_of(const oracle_cfg *cfg, float x, float y, float z, float *phi_out);
 
+// Window of one outer hit (pair stage) or one pair (prediction; store != 0
+// selects the store-side variant).  out: zLow, zHigh, phiLow, phiHigh, wrap,
+// zLowSector, zHighSector, phiLowSector, phiHighSector.
+void oracle_pair_window(const oracle_cfg *cfg, uint32_t p, float x, float y, float z, float *out);
+void oracle_predict_window(const oracle_cfg *cfg, uint32_t lt, int store, const float *p1,
+                           const float *p2, float minR, float maxR, float *out);
+// Filter quantities of one triplet: out = theta ratio, |dphi|, tip^2, circle radius, pass.
+void oracle_filter_eval(const oracle_cfg *cfg, uint32_t lt, const float *h1, const float *h2,
+                        const float *h3, float *out);
+
 #ifdef __cplusplus
 }
 #endif

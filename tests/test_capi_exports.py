@@ -1,0 +1,71 @@
+"""The C ABI library loads and exports every symbol include/trax_hip.h declares
+(no compute calls: these run without a GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from trax_amd import events as ev
+from trax_amd import runtime
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "trax_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(trax_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_the_abi():
+    names = _declared()
+    assert "trax_seed_chain" in names and "trax_prefix_sum" in names
+    assert set(names) == set(runtime.SIGNATURES), set(names) ^ set(runtime.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = runtime.lib()
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_host_only_entry_points():
+    lib = runtime.lib()
+    assert lib.trax_abi_version() == 1
+    for pt in (0.3, 0.5, 1.0):
+        assert np.float32(lib.trax_min_radius_curvature(pt)) == ev.min_radius_curvature(pt)
+    # 1 GeV at Bz = 3.8112 T: 87.52 cm (TripletConfiguration.cpp:145-152)
+    assert abs(lib.trax_min_radius_curvature(1.0) - 87.522) < 1e-3
+
+
+def test_workspace_layout_sizes():
+    lib = runtime.lib()
+    sc = ev.config_workload(2, 4).seed_config
+    cfg = runtime.seed_config_c(sc)
+    ws = runtime.WorkspaceC()
+    n = lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, None, ctypes.byref(ws))
+    need = (4 * 2 + 1) * 4 + 10001 * 4 + 50001 * 4
+    assert need <= n <= need + 4 * 256
+    assert ws.max_events == 4 and ws.max_hits == 10000 and ws.pair_capacity == 50000
+    base = 1 << 20
+    lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, ctypes.c_void_p(base),
+                              ctypes.byref(ws))
+    for p in (ws.work_offsets, ws.pair_counts, ws.cand_counts):
+        assert p % 256 == 0 and base <= p < base + n
+    assert not ws.grid_counters  # 65 x 129 cells fit in LDS
+    big = ev.config_workload(2, 4).seed_config
+    big.n_z, big.n_phi = 128, 256
+    cfgb = runtime.seed_config_c(big)
+    nb = lib.trax_workspace_layout(ctypes.byref(cfgb), 4, 10000, 50000, ctypes.c_void_p(base),
+                                   ctypes.byref(ws))
+    assert ws.grid_counters and nb >= n + 4 * 4 * 129 * 257 * 4
+
+
+def test_context_requires_a_device():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(runtime.TraxError):
+        runtime.Context(0)

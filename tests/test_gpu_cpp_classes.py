@@ -1,0 +1,42 @@
+"""The C++ algorithm classes (GridBuilder, PairGeneratorBeamspot,
+TripletThetaPhiPredictor, TripletThetaPhiFilter) driven by trax_seed, in the
+reference's stage order, against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import cand_rows, rows_sorted, with_segment, written_pairs
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "trax_amd", "_build", "trax_seed")
+
+
+@pytest.mark.parametrize("config_id,n_events", [(0, 1), (3, 2)])
+def test_cpp_classes_match_oracle(tmp_path, config_id, n_events):
+    from oracle import bridge
+    from trax_amd import eventfile
+    from trax_amd import events as ev
+    w = ev.config_workload(config_id, n_events)
+    sc, batch = w.seed_config, ev.make_batch(w)
+    fin, fout = str(tmp_path / "events.bin"), str(tmp_path / "results.bin")
+    eventfile.write_events(fin, sc, batch)
+    print(f"trax_seed config {config_id}, {n_events} events", flush=True)
+    r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = eventfile.read_results(fout, batch.n_events, len(sc.triplets))
+    o = bridge.run_chain(sc, batch)
+    assert np.array_equal(g["orig_index"], o["orig_index"])
+    assert np.array_equal(g["grid"], o["grid"])
+    assert np.array_equal(g["pair_offsets"], o["pair_offsets"])
+    assert np.array_equal(rows_sorted(with_segment(g["pairs"], g["pair_offsets"])),
+                          rows_sorted(with_segment(written_pairs(o["pairs"]), o["pair_offsets"])))
+    assert np.array_equal(g["cand_offsets"], o["cand_offsets"])
+    assert np.array_equal(rows_sorted(cand_rows(g["pairs"], g["cands"], g["cand_offsets"])),
+                          rows_sorted(cand_rows(o["pairs"], o["cands"], o["cand_offsets"])))
+    assert np.array_equal(g["tracklet_offsets"], o["tracklet_offsets"])
+    assert np.array_equal(rows_sorted(with_segment(g["tracklets"], g["tracklet_offsets"])),
+                          rows_sorted(with_segment(o["tracklets"], o["tracklet_offsets"])))

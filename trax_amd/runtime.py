@@ -379,3 +379,71 @@ def fp32_eval(ctx: Context, op: str, a, b=None, stream=None):
                                      _vp(out.data_ptr()), a.numel(), stream_ptr(stream)),
               "trax_fp32_eval")
     return out
+
+
+# ------------------------------------------------------------ stage-by-stage
+def _stage_args(bufs: ChainBuffers):
+    return bufs.cfg, bufs.sorted_c(), bufs.outputs_c()
+
+
+def grid_build(ctx: Context, bufs: ChainBuffers, db: DeviceBatch, stream=None) -> None:
+    """GridBuilder::run (GridBuilder.h:42) on the device batch."""
+    hin, lay, hs = db.hits_c(), db.layout_c(), bufs.sorted_c()
+    ctx.check(ctx.lib.trax_grid_build(ctx.h, C.byref(bufs.cfg), C.byref(hin), C.byref(lay),
+                                      C.byref(hs), _vp(_ptr(bufs.grid)), C.byref(bufs.ws),
+                                      stream_ptr(stream)), "trax_grid_build")
+
+
+def _stage(ctx, fn, name, bufs, n_events, with_grid=True, stream=None):
+    cfg, hs, out = _stage_args(bufs)
+    bufs._keep = (hs, out)
+    if with_grid:
+        rc = fn(ctx.h, C.byref(cfg), n_events, C.byref(hs), _vp(_ptr(bufs.grid)), C.byref(bufs.ws),
+                C.byref(out), stream_ptr(stream))
+    else:
+        rc = fn(ctx.h, C.byref(cfg), n_events, C.byref(hs), C.byref(out), stream_ptr(stream))
+    ctx.check(rc, name)
+
+
+def pairs_count(ctx, bufs, n_events, stream=None):
+    _stage(ctx, ctx.lib.trax_pairs_count, "trax_pairs_count", bufs, n_events, stream=stream)
+
+
+def pairs_store(ctx, bufs, n_events, stream=None):
+    _stage(ctx, ctx.lib.trax_pairs_store, "trax_pairs_store", bufs, n_events, stream=stream)
+
+
+def predict_count(ctx, bufs, n_events, stream=None):
+    _stage(ctx, ctx.lib.trax_predict_count, "trax_predict_count", bufs, n_events, stream=stream)
+
+
+def predict_store(ctx, bufs, n_events, stream=None):
+    _stage(ctx, ctx.lib.trax_predict_store, "trax_predict_store", bufs, n_events, stream=stream)
+
+
+def triplet_filter(ctx, bufs, n_events, stream=None):
+    _stage(ctx, ctx.lib.trax_triplet_filter, "trax_triplet_filter", bufs, n_events,
+           with_grid=False, stream=stream)
+
+
+def run_plain_event(ctx: Context, bufs: ChainBuffers, db: DeviceBatch) -> int:
+    """The plain method for one resident event: every stage is its own set of
+    launches and the host reads each stage's total before the next store, as
+    the reference does (PairGeneratorBeamspot.cpp:120, TripletThetaPhi-
+    Predictor.cpp:59, TripletThetaPhiFilter.cpp:82).  Returns the triplet count."""
+    torch = _torch()
+    host = torch.empty(4, dtype=torch.int32, pin_memory=True)
+    grid_build(ctx, bufs, db)
+    pairs_count(ctx, bufs, db.n_events)
+    host.copy_(bufs.totals)  # blocking read of the pair total
+    if int(host[0]) > bufs.pair_capacity:
+        raise TraxError("pair capacity exceeded in the plain path")
+    pairs_store(ctx, bufs, db.n_events)
+    predict_count(ctx, bufs, db.n_events)
+    host.copy_(bufs.totals)  # blocking read of the candidate total
+    if int(host[1]) > bufs.cand_capacity:
+        raise TraxError("candidate capacity exceeded in the plain path")
+    predict_store(ctx, bufs, db.n_events)
+    triplet_filter(ctx, bufs, db.n_events)
+    host.copy_(bufs.totals)  # blocking read of the triplet total
+    return int(host[2])
