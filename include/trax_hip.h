@@ -123,11 +123,21 @@ typedef struct {
   uint32_t *totals;
 } trax_seed_outputs;
 
+/* Launch shape of the kernels, per context (set by the autotuner). */
+typedef struct {
+  uint32_t grid_waves;    /* waves per grid-build block: 1, 2, 4 or 8 (default 4) */
+  uint32_t pairs_block;   /* threads per block, pair kernels: 64, 128, 256, 512 (default 256) */
+  uint32_t predict_block; /* threads per block, prediction kernels (default 256) */
+  uint32_t scan_tile;     /* elements per scan tile: 4096, 8192 or 16384 (default 8192) */
+} trax_launch_config;
+
 /* ---------------------------------------------------------------- context */
 int trax_abi_version(void);
 int trax_ctx_create(int device, trax_ctx **out);
 int trax_ctx_destroy(trax_ctx *ctx);
 const char *trax_ctx_last_error(const trax_ctx *ctx);
+int trax_ctx_set_launch(trax_ctx *ctx, const trax_launch_config *cfg);
+int trax_ctx_get_launch(const trax_ctx *ctx, trax_launch_config *cfg);
 /* Synchronous read of the device status bits; clear != 0 resets them. */
 int trax_ctx_flags(trax_ctx *ctx, void *stream, uint32_t *flags_out, int clear);
 

@@ -1,0 +1,123 @@
+"""Autotuner for the seeding chain.
+
+Searches, per workload, the grid binning (phi x z sectors) and the launch
+shape of every stage (waves per grid-build block, threads per block of the
+pair and prediction kernels, scan tile size), timing each stage with HIP
+events on the launch stream.  Stages are tuned independently for a given
+binning (their launch shapes do not change their inputs); binnings are
+compared on the whole chain with each stage at its best shape.  Results are
+written as JSON and can be loaded into a context with `apply`.
+
+The binning changes the grid and so, in edge cases, the pair and triplet
+sets (the reference's window walk depends on the cells); the parity tests
+run the oracle at the tuned binning.
+"""
+from __future__ import annotations
+
+import json
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+from . import events as ev
+from . import runtime as rt
+
+WAVES = (1, 2, 4, 8)
+BLOCKS = (64, 128, 256, 512)
+SCAN_TILES = (4096, 8192, 16384)
+PHI_BINS = (64, 128, 256)
+Z_BINS = (32, 64, 128)
+
+
+@dataclass
+class TuneResult:
+    workload: str
+    n_phi: int
+    n_z: int
+    launch: Dict[str, int]
+    chain_ms: float
+    stage_ms: Dict[str, float]
+    table: List[dict] = field(default_factory=list)
+
+    def to_json(self) -> str:
+        return json.dumps(self.__dict__, indent=1)
+
+
+def _time(fn, reps, torch) -> float:
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps: int = 3):
+    """Best launch shape per stage at this binning; returns (launch, stage_ms, chain_ms)."""
+    import torch
+    base = dict(grid_waves=4, pairs_block=256, predict_block=256, scan_tile=8192)
+    ctx.set_launch(**base)
+    bufs, db = rt.run_batch(ctx, sc, batch)
+    E = db.n_events
+    best = dict(base)
+    stage_ms: Dict[str, float] = {}
+
+    def sweep(key, values, fn, name):
+        times = {}
+        for v in values:
+            cfg = dict(best)
+            cfg[key] = v
+            ctx.set_launch(**cfg)
+            times[v] = _time(fn, reps, torch)
+        v = min(times, key=times.get)
+        best[key] = v
+        stage_ms[name] = times[v]
+        return times
+
+    sweep("grid_waves", WAVES, lambda: rt.grid_build(ctx, bufs, db), "grid_build")
+    sweep("scan_tile", SCAN_TILES, lambda: rt.pairs_count(ctx, bufs, E), "pairs_count_scan")
+    sweep("pairs_block", BLOCKS, lambda: (rt.pairs_count(ctx, bufs, E), rt.pairs_store(ctx, bufs, E)),
+          "pairs")
+    sweep("predict_block", BLOCKS,
+          lambda: (rt.predict_count(ctx, bufs, E), rt.predict_store(ctx, bufs, E)), "predict")
+    ctx.set_launch(**best)
+    stage_ms["filter"] = _time(lambda: rt.triplet_filter(ctx, bufs, E), reps, torch)
+    chain_ms = _time(lambda: rt.seed_chain(ctx, bufs, db), reps, torch)
+    if ctx.flags():
+        raise rt.TraxError("overflow during tuning")
+    return best, stage_ms, chain_ms
+
+
+def tune(ctx: rt.Context, w: ev.Workload, batch: Optional[ev.EventBatch] = None,
+         phi_bins: Sequence[int] = PHI_BINS, z_bins: Sequence[int] = Z_BINS,
+         reps: int = 3, log=None) -> TuneResult:
+    batch = batch if batch is not None else ev.make_batch(w)
+    table = []
+    best: Optional[Tuple] = None
+    for n_phi in phi_bins:
+        for n_z in z_bins:
+            sc = ev.SeedConfig(n_z=n_z, n_phi=n_phi, min_radius=w.seed_config.min_radius,
+                               max_radius=w.seed_config.max_radius, triplets=w.seed_config.triplets,
+                               min_pt=w.seed_config.min_pt)
+            t0 = time.perf_counter()
+            launch, st, chain = tune_binning(ctx, sc, batch, reps)
+            row = {"n_phi": n_phi, "n_z": n_z, "chain_ms": chain, "stage_ms": st, "launch": launch,
+                   "tune_s": time.perf_counter() - t0}
+            table.append(row)
+            if log:
+                log(f"phi {n_phi} z {n_z}: chain {chain:.3f} ms {launch}")
+            if best is None or chain < best[0]:
+                best = (chain, n_phi, n_z, launch, st)
+    chain, n_phi, n_z, launch, st = best
+    return TuneResult(w.name, n_phi, n_z, launch, chain, st, table)
+
+
+def apply(ctx: rt.Context, w: ev.Workload, result: TuneResult) -> ev.Workload:
+    """Set the tuned launch shape on ctx and the tuned binning on the workload."""
+    ctx.set_launch(**result.launch)
+    w.seed_config.n_phi, w.seed_config.n_z = result.n_phi, result.n_z
+    return w

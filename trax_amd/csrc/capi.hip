@@ -21,6 +21,7 @@ struct trax_ctx {
   uint32_t* flags = nullptr;
   unsigned long long* status = nullptr;
   size_t lds_max = 65536;
+  trax_launch_config launch = {4, 256, 256, 8192};
 };
 
 namespace {
@@ -139,6 +140,8 @@ int check_ws(trax_ctx* ctx, const trax_workspace* ws, uint32_t n_events, uint32_
   return TRAX_OK;
 }
 
+int scan_sub(const trax_ctx* ctx) { return (int)(ctx->launch.scan_tile / 1024u); }
+
 uint64_t pair_cap(const trax_workspace* ws, const trax_seed_outputs* out) {
   return out->pair_capacity < ws->pair_capacity ? out->pair_capacity : ws->pair_capacity;
 }
@@ -194,6 +197,25 @@ extern "C" int trax_ctx_destroy(trax_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->block) (void)hipFree(ctx->block);
   delete ctx;
+  return TRAX_OK;
+}
+
+extern "C" int trax_ctx_set_launch(trax_ctx* ctx, const trax_launch_config* cfg) {
+  if (!ctx || !cfg) return TRAX_ERR_INVALID;
+  const uint32_t w = cfg->grid_waves;
+  if (!(w == 1 || w == 2 || w == 4 || w == 8)) return fail(ctx, TRAX_ERR_INVALID, "grid_waves must be 1, 2, 4 or 8");
+  for (uint32_t b : {cfg->pairs_block, cfg->predict_block})
+    if (!(b == 64 || b == 128 || b == 256 || b == 512))
+      return fail(ctx, TRAX_ERR_INVALID, "block sizes must be 64, 128, 256 or 512");
+  if (!(cfg->scan_tile == 4096 || cfg->scan_tile == 8192 || cfg->scan_tile == 16384))
+    return fail(ctx, TRAX_ERR_INVALID, "scan_tile must be 4096, 8192 or 16384");
+  ctx->launch = *cfg;
+  return TRAX_OK;
+}
+
+extern "C" int trax_ctx_get_launch(const trax_ctx* ctx, trax_launch_config* cfg) {
+  if (!ctx || !cfg) return TRAX_ERR_INVALID;
+  *cfg = ctx->launch;
   return TRAX_OK;
 }
 
@@ -268,7 +290,7 @@ extern "C" int trax_prefix_sum_dev(trax_ctx* ctx, const uint32_t* d_in, uint32_t
     return fail(ctx, TRAX_ERR_INVALID, "scan buffers must be 16-byte aligned");
   if (n_max > 0xFFFFFFFFull) return fail(ctx, TRAX_ERR_INVALID, "scan longer than 2^32 elements");
   TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(d_in, d_out, n_max, d_n, n_add, ctx->ctrl, ctx->status,
-                                            static_cast<hipStream_t>(stream)));
+                                            static_cast<hipStream_t>(stream), scan_sub(ctx)));
   return TRAX_OK;
 }
 
@@ -309,6 +331,7 @@ extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const
   gl.flags = ctx->flags;
   gl.counters = nullptr;
   gl.use_lds = (uint64_t)sp.cells * 4u <= ctx->lds_max;
+  gl.waves = (int)ctx->launch.grid_waves;
   if (!gl.use_lds) {
     if (!ws || !ws->grid_counters || ws->max_events < layout->n_events)
       return fail(ctx, TRAX_ERR_INVALID, "grid too large for LDS and no workspace grid counters");
@@ -334,9 +357,10 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   const trax::SortedHits h = sorted_view(sorted);
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
   TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
-                                         ws->max_hits, false, ctx->flags, s));
+                                         ws->max_hits, false, ctx->flags, ctx->launch.pairs_block, s));
   TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
-                                            ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s));
+                                            ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s,
+                                            scan_sub(ctx)));
   // pair_offsets[k] = scanned count at the first outer hit of segment k
   TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->pair_counts, ws->max_hits, ws->work_offsets + K,
                                                   ws->work_offsets, K + 1, out->pair_offsets,
@@ -357,7 +381,7 @@ extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint
   TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                          ws->pair_counts, reinterpret_cast<uint2*>(out->pairs),
                                          out->pair_capacity, ws->max_hits, true, ctx->flags,
-                                         static_cast<hipStream_t>(stream)));
+                                         ctx->launch.pairs_block, static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
 
@@ -379,9 +403,9 @@ extern "C" int trax_predict_count(trax_ctx* ctx, const trax_seed_config* cfg, ui
   TRAX_HIP_CHECK(ctx, trax::launch_predict(sp, sorted_view(sorted), d_grid,
                                            reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
                                            out->totals + 0, pc, ws->cand_counts, nullptr, 0, false,
-                                           ctx->flags, s));
+                                           ctx->flags, ctx->launch.predict_block, s));
   TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->cand_counts, ws->cand_counts, pc + 1, out->totals + 0, 1,
-                                            ctx->ctrl, ctx->status, s));
+                                            ctx->ctrl, ctx->status, s, scan_sub(ctx)));
   TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->cand_counts, (uint32_t)pc, out->totals + 0,
                                                   out->pair_offsets, K + 1, out->cand_offsets,
                                                   out->totals + 1, s));
@@ -403,7 +427,8 @@ extern "C" int trax_predict_store(trax_ctx* ctx, const trax_seed_config* cfg, ui
                                            reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
                                            out->totals + 0, pair_cap(ws, out), ws->cand_counts,
                                            reinterpret_cast<uint2*>(out->cands), out->cand_capacity, true,
-                                           ctx->flags, static_cast<hipStream_t>(stream)));
+                                           ctx->flags, ctx->launch.predict_block,
+                                           static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
 

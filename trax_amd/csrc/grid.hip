@@ -49,14 +49,13 @@ struct GridArgs {
   uint32_t* __restrict__ flags;
 };
 
-// In-block exclusive scan of cnt[0..C) (256 threads, 4 consecutive cells per
-// thread per 1024-cell step).  `cnt` is LDS or global memory.
-template <typename Ptr>
-__device__ __forceinline__ void block_exclusive_scan_cells(Ptr cnt, uint32_t C, uint32_t* s_wave,
-                                                           uint32_t* s_carry) {
+// In-block exclusive scan of cnt[0..C) (64*NW threads, 4 consecutive cells
+// per thread per step).  `cnt` is LDS or global memory.
+template <int NW, typename Ptr>
+__device__ __forceinline__ void block_exclusive_scan_cells(Ptr cnt, uint32_t C, uint32_t* s_wave) {
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   uint32_t carry = 0;
-  for (uint32_t base = 0; base < C; base += 1024u) {
+  for (uint32_t base = 0; base < C; base += 256u * NW) {
     uint32_t c0 = base + tid * 4u;
     uint32_t v0 = c0 < C ? cnt[c0] : 0u;
     uint32_t v1 = c0 + 1 < C ? cnt[c0 + 1] : 0u;
@@ -68,7 +67,7 @@ __device__ __forceinline__ void block_exclusive_scan_cells(Ptr cnt, uint32_t C, 
     __syncthreads();
     uint32_t wp = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
+    for (int w = 0; w < NW; ++w) {
       uint32_t c = s_wave[w];
       wp += (w < (int)wave) ? c : 0u;
       tot += c;
@@ -81,15 +80,14 @@ __device__ __forceinline__ void block_exclusive_scan_cells(Ptr cnt, uint32_t C, 
     carry += tot;
     __syncthreads();
   }
-  (void)s_carry;
 }
 
-template <bool LDS_CELLS>
-__global__ __launch_bounds__(256) void k_grid_build(SeedParams sp, GridArgs a) {
+template <bool LDS_CELLS, int NW>
+__global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs a) {
+  constexpr uint32_t BLOCK = 64u * NW;
   extern __shared__ uint32_t s_dyn[];
-  __shared__ uint32_t s_wave[4];
-  __shared__ uint32_t s_carry;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  __shared__ uint32_t s_wave[NW];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t C = sp.cells, L = sp.n_layers;
   const uint32_t nseg = sp.n_events * L;
   for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
@@ -102,22 +100,22 @@ __global__ __launch_bounds__(256) void k_grid_build(SeedParams sp, GridArgs a) {
       nh = 0;
     }
     uint32_t* cnt = LDS_CELLS ? s_dyn : (a.counters + (uint64_t)seg * C);
-    for (uint32_t c = tid; c < C; c += 256u) cnt[c] = 0u;
+    for (uint32_t c = tid; c < C; c += BLOCK) cnt[c] = 0u;
     __syncthreads();
     // histogram (gridCount, GridBuilder.h:79-85)
-    for (uint32_t i = tid; i < nh; i += 256u) {
+    for (uint32_t i = tid; i < nh; i += BLOCK) {
       float phi;
       uint32_t cell = grid_cell(sp, a.ix[base + i], a.iy[base + i], a.iz[base + i], &phi);
       atomicAdd(&cnt[cell], 1u);
     }
     __syncthreads();
-    block_exclusive_scan_cells(cnt, C, s_wave, &s_carry);
+    block_exclusive_scan_cells<NW>(cnt, C, s_wave);
     // cell starts in the global sorted array (the reference's scanned grid)
     uint32_t* g = a.grid + (uint64_t)seg * C;
-    for (uint32_t c = tid; c < C; c += 256u) g[c] = base + cnt[c];
+    for (uint32_t c = tid; c < C; c += BLOCK) g[c] = base + cnt[c];
     __syncthreads();
     // stable scatter (gridStore, GridBuilder.h:238-253, with a stable rank)
-    for (uint32_t s0 = 0; s0 < nh; s0 += 256u) {
+    for (uint32_t s0 = 0; s0 < nh; s0 += BLOCK) {
       const uint32_t i = s0 + tid;
       const bool valid = i < nh;
       float x = 0.f, y = 0.f, z = 0.f, phi = 0.f;
@@ -137,7 +135,7 @@ __global__ __launch_bounds__(256) void k_grid_build(SeedParams sp, GridArgs a) {
       const uint32_t rank = lanes_below(match);
       const uint32_t count = (uint32_t)__popcll(match);
       uint32_t pos = 0;
-      for (uint32_t w = 0; w < 4u; ++w) {
+      for (uint32_t w = 0; w < (uint32_t)NW; ++w) {
         if (wave == w && valid) {
           pos = cnt[cell] + rank;
           if (rank == 0) cnt[cell] = pos + count;
@@ -161,6 +159,18 @@ __global__ __launch_bounds__(256) void k_grid_build(SeedParams sp, GridArgs a) {
   }
 }
 
+template <bool L>
+static hipError_t grid_launch_nw(int nw, uint32_t blocks, size_t lds, hipStream_t stream,
+                                 const SeedParams& sp, const GridArgs& a) {
+  switch (nw) {
+    case 1: hipLaunchKernelGGL((k_grid_build<L, 1>), dim3(blocks), dim3(64), lds, stream, sp, a); break;
+    case 2: hipLaunchKernelGGL((k_grid_build<L, 2>), dim3(blocks), dim3(128), lds, stream, sp, a); break;
+    case 8: hipLaunchKernelGGL((k_grid_build<L, 8>), dim3(blocks), dim3(512), lds, stream, sp, a); break;
+    default: hipLaunchKernelGGL((k_grid_build<L, 4>), dim3(blocks), dim3(256), lds, stream, sp, a); break;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream) {
   GridArgs a;
   a.ix = gl.ix; a.iy = gl.iy; a.iz = gl.iz;
@@ -176,18 +186,21 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
   const uint32_t nseg = sp.n_events * sp.n_layers;
   if (nseg == 0) return hipSuccess;
   const uint32_t blocks = nseg < 65535u ? nseg : 65535u;
-  const size_t lds = (size_t)sp.cells * 4u;
-  if (gl.use_lds) {
-    hipLaunchKernelGGL(k_grid_build<true>, dim3(blocks), dim3(256), lds, stream, sp, a);
-  } else {
-    hipLaunchKernelGGL(k_grid_build<false>, dim3(blocks), dim3(256), 0, stream, sp, a);
-  }
-  return hipGetLastError();
+  if (gl.use_lds) return grid_launch_nw<true>(gl.waves, blocks, (size_t)sp.cells * 4u, stream, sp, a);
+  return grid_launch_nw<false>(gl.waves, blocks, 0, stream, sp, a);
 }
 
 hipError_t grid_build_set_lds_limit(size_t bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_grid_build<true>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  hipError_t e = hipSuccess;
+  const void* fns[] = {reinterpret_cast<const void*>(&k_grid_build<true, 1>),
+                       reinterpret_cast<const void*>(&k_grid_build<true, 2>),
+                       reinterpret_cast<const void*>(&k_grid_build<true, 4>),
+                       reinterpret_cast<const void*>(&k_grid_build<true, 8>)};
+  for (const void* f : fns) {
+    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (r != hipSuccess) e = r;
+  }
+  return e;
 }
 
 }  // namespace trax

@@ -16,7 +16,7 @@ constexpr uint64_t kStatusWords = 1ull << 20;
 uint64_t scan_tile_elems();
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
                            uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
-                           hipStream_t stream);
+                           hipStream_t stream, int sub = 8);
 
 struct GridLaunch {
   const float *ix, *iy, *iz;
@@ -30,6 +30,7 @@ struct GridLaunch {
   uint32_t* counters;
   uint32_t* flags;
   bool use_lds;
+  int waves;  // waves per block: 1, 2, 4 or 8
 };
 hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream);
 hipError_t grid_build_set_lds_limit(size_t bytes);
@@ -38,14 +39,16 @@ hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t
                             hipStream_t stream);
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                         const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
-                        uint32_t max_work, bool store, uint32_t* flags, hipStream_t stream);
+                        uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
+                        hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
                                  hipStream_t stream);
 hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                           const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                           uint64_t pair_capacity, uint32_t* counts, uint2* cands,
-                          uint64_t cand_capacity, bool store, uint32_t* flags, hipStream_t stream);
+                          uint64_t cand_capacity, bool store, uint32_t* flags, uint32_t block,
+                          hipStream_t stream);
 hipError_t launch_filter(const SeedParams& sp, const SortedHits& h, const uint2* pairs,
                          const uint2* cands, const uint32_t* cand_off, const uint32_t* d_ncands,
                          uint64_t cand_capacity, uint32_t* t1, uint32_t* t2, uint32_t* t3,

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_pair_plan(SeedParams sp, const uint32_t
 }
 
 template <bool STORE>
-__global__ __launch_bounds__(256) void k_pairs(SeedParams sp, SortedHits h,
+__global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
                                                const uint32_t* __restrict__ grid,
                                                const uint32_t* __restrict__ work_off,
                                                uint32_t* __restrict__ counts,  // count: out; store: scanned
@@ -202,15 +202,16 @@ hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t
 
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                         const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
-                        uint32_t max_work, bool store, uint32_t* flags, hipStream_t stream) {
-  uint32_t blocks = (max_work + 255u) / 256u;
+                        uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
+                        hipStream_t stream) {
+  uint32_t blocks = (max_work + block - 1) / block;
   if (blocks == 0) blocks = 1;
-  if (blocks > 8192u) blocks = 8192u;
+  if (blocks > 16384u) blocks = 16384u;
   if (store)
-    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, stream, sp, h, grid, work_off,
+    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(block), 0, stream, sp, h, grid, work_off,
                        counts, pairs, capacity, flags);
   else
-    hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, stream, sp, h, grid, work_off,
+    hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(block), 0, stream, sp, h, grid, work_off,
                        counts, pairs, capacity, flags);
   return hipGetLastError();
 }

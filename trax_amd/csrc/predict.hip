@@ -86,7 +86,7 @@ __device__ __forceinline__ PredWindow predict_window(const SeedParams& sp, uint3
 }
 
 template <bool STORE>
-__global__ __launch_bounds__(256) void k_predict(SeedParams sp, SortedHits h,
+__global__ __launch_bounds__(512) void k_predict(SeedParams sp, SortedHits h,
                                                  const uint32_t* __restrict__ grid,
                                                  const uint2* __restrict__ pairs,
                                                  const uint32_t* __restrict__ pair_off,
@@ -169,14 +169,15 @@ __global__ __launch_bounds__(256) void k_predict(SeedParams sp, SortedHits h,
 hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                           const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                           uint64_t pair_capacity, uint32_t* counts, uint2* cands,
-                          uint64_t cand_capacity, bool store, uint32_t* flags, hipStream_t stream) {
-  uint64_t b = (pair_capacity + 255u) / 256u;
-  uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 16384u ? 16384u : b));
+                          uint64_t cand_capacity, bool store, uint32_t* flags, uint32_t block,
+                          hipStream_t stream) {
+  uint64_t b = (pair_capacity + block - 1) / block;
+  uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 32768u ? 32768u : b));
   if (store)
-    hipLaunchKernelGGL(k_predict<true>, dim3(blocks), dim3(256), 0, stream, sp, h, grid, pairs,
+    hipLaunchKernelGGL(k_predict<true>, dim3(blocks), dim3(block), 0, stream, sp, h, grid, pairs,
                        pair_off, d_npairs, pair_capacity, counts, cands, cand_capacity, flags);
   else
-    hipLaunchKernelGGL(k_predict<false>, dim3(blocks), dim3(256), 0, stream, sp, h, grid, pairs,
+    hipLaunchKernelGGL(k_predict<false>, dim3(blocks), dim3(block), 0, stream, sp, h, grid, pairs,
                        pair_off, d_npairs, pair_capacity, counts, cands, cand_capacity, flags);
   return hipGetLastError();
 }

@@ -114,20 +114,30 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
   }
 }
 
-static constexpr int kScanSub = 8;
-static constexpr uint64_t kScanTile = 256ull * 4ull * kScanSub;
+uint64_t scan_tile_elems() { return 256ull * 4ull * 8ull; }
 
-uint64_t scan_tile_elems() { return kScanTile; }
-
-hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
-                           uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
-                           hipStream_t stream) {
-  uint64_t blocks = (n_max + kScanTile - 1) / kScanTile;
+template <int SUB>
+static hipError_t scan_launch(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
+                              uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
+                              hipStream_t stream) {
+  constexpr uint64_t tile = 256ull * 4ull * SUB;
+  uint64_t blocks = (n_max + tile - 1) / tile;
   if (blocks == 0) blocks = 1;
   if (blocks > kPersistentBlocks) blocks = kPersistentBlocks;
-  hipLaunchKernelGGL(k_scan_u32<kScanSub>, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out,
-                     n_max, d_n, n_add, ctrl, status);
+  hipLaunchKernelGGL(k_scan_u32<SUB>, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out, n_max,
+                     d_n, n_add, ctrl, status);
   return hipGetLastError();
+}
+
+// sub: uint4 loads per thread per tile (tile = 1024 * sub elements).
+hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
+                           uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
+                           hipStream_t stream, int sub) {
+  switch (sub) {
+    case 4: return scan_launch<4>(in, out, n_max, d_n, n_add, ctrl, status, stream);
+    case 16: return scan_launch<16>(in, out, n_max, d_n, n_add, ctrl, status, stream);
+    default: return scan_launch<8>(in, out, n_max, d_n, n_add, ctrl, status, stream);
+  }
 }
 
 }  // namespace trax

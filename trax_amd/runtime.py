@@ -62,6 +62,11 @@ class WorkspaceC(C.Structure):
                 ("pair_capacity", C.c_uint64)]
 
 
+class LaunchC(C.Structure):
+    _fields_ = [("grid_waves", C.c_uint32), ("pairs_block", C.c_uint32),
+                ("predict_block", C.c_uint32), ("scan_tile", C.c_uint32)]
+
+
 class OutputsC(C.Structure):
     _fields_ = [("pairs", _vp), ("pair_capacity", C.c_uint64), ("pair_offsets", _vp),
                 ("cands", _vp), ("cand_capacity", C.c_uint64), ("cand_offsets", _vp),
@@ -77,6 +82,8 @@ SIGNATURES = {
     "trax_ctx_destroy": (C.c_int, [_vp]),
     "trax_ctx_last_error": (C.c_char_p, [_vp]),
     "trax_ctx_flags": (C.c_int, [_vp, _vp, _u32p, C.c_int]),
+    "trax_ctx_set_launch": (C.c_int, [_vp, _P(LaunchC)]),
+    "trax_ctx_get_launch": (C.c_int, [_vp, _P(LaunchC)]),
     "trax_min_radius_curvature": (C.c_float, [C.c_float]),
     "trax_workspace_layout": (C.c_uint64, [_P(SeedConfigC), C.c_uint32, C.c_uint32, C.c_uint64, _vp,
                                            _P(WorkspaceC)]),
@@ -180,6 +187,15 @@ class Context:
         if rc != 0:
             msg = self.lib.trax_ctx_last_error(self.h)
             raise TraxError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def set_launch(self, grid_waves=4, pairs_block=256, predict_block=256, scan_tile=8192):
+        cfg = LaunchC(grid_waves, pairs_block, predict_block, scan_tile)
+        self.check(self.lib.trax_ctx_set_launch(self.h, C.byref(cfg)), "trax_ctx_set_launch")
+
+    def get_launch(self) -> dict:
+        cfg = LaunchC()
+        self.check(self.lib.trax_ctx_get_launch(self.h, C.byref(cfg)), "trax_ctx_get_launch")
+        return {k: getattr(cfg, k) for k, _ in LaunchC._fields_}
 
     def flags(self, stream=None, clear: bool = True) -> int:
         v = C.c_uint32(0)

@@ -1,0 +1,212 @@
+"""Continuous-batching seeding server.
+
+Events arrive as a stream (e.g. Poisson).  The server owns a fixed set of
+slots; each slot is a HIP stream with its own trax context and resident
+device buffers (hits, sorted hits, grid, pairs, candidates, triplets) sized
+so that all slots together stay inside the device-memory budget.  Whenever a
+slot is free, the events waiting at that moment are admitted into it as one
+dynamic batch (1..max_batch events, limited by the slot's hit capacity), and
+the whole chain runs on that slot's stream while other slots are still busy,
+so batches overlap on the device.  Latency of an event = completion of its
+batch - its arrival; results are split back per event.
+
+An event's triplets do not depend on the batch it is served in: the chain
+treats every (event, layer) segment independently.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import events as ev
+from . import runtime as rt
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds, relative to the start of the run
+    batch: ev.EventBatch  # one event
+    done: float = -1.0
+    triplets: Optional[np.ndarray] = None  # (n, 3) input-hit indices within the event
+
+
+class Slot:
+    def __init__(self, sc: ev.SeedConfig, max_events: int, max_hits: int, pair_cap: int,
+                 cand_cap: int, trk_cap: int, device="cuda"):
+        import torch
+        self.torch = torch
+        self.stream = torch.cuda.Stream(device=device)
+        self.ctx = rt.Context(torch.device(device).index or 0)
+        self.bufs = rt.ChainBuffers(sc, max_events, max_hits, pair_cap, cand_cap, trk_cap, device)
+        n = max_hits
+        self.dev = {k: torch.empty(max(n, 1), dtype=torch.float32 if k in "xyz" else torch.int32,
+                                   device=device) for k in ("x", "y", "z", "layer", "det_id", "hit_id", "evt")}
+        self.dev_layout = {k: torch.empty(max_events * max(1, sc.n_layers), dtype=torch.int32, device=device)
+                           for k in ("event_offset", "layer_nhits", "layer_offset")}
+        self.host = {k: torch.empty_like(v, device="cpu").pin_memory() for k, v in self.dev.items()}
+        self.host_layout = {k: torch.empty_like(v, device="cpu").pin_memory()
+                            for k, v in self.dev_layout.items()}
+        self.requests: List[Request] = []
+        self.done_event = None
+        self.db: Optional[rt.DeviceBatch] = None
+        self.batch: Optional[ev.EventBatch] = None
+
+    @property
+    def busy(self) -> bool:
+        return bool(self.requests)
+
+    def nbytes(self) -> int:
+        b = self.bufs
+        tot = sum(t.numel() * t.element_size() for t in b.sorted.values())
+        tot += b.grid.numel() * 4 + b.ws_mem.numel() + b.pairs.numel() * 4 + b.cands.numel() * 4
+        tot += sum(t.numel() * 4 for t in b.t) + sum(t.numel() * t.element_size() for t in self.dev.values())
+        return int(tot)
+
+    def launch(self, reqs: List[Request]) -> None:
+        torch = self.torch
+        batch = ev.concat_events([r.batch for r in reqs])
+        n = batch.n_hits
+        for k, col in zip(("x", "y", "z", "layer", "det_id", "hit_id", "evt"), batch.columns()):
+            h = self.host[k]
+            src = torch.from_numpy(col.view(np.int32) if col.dtype == np.uint32 else col)
+            h[:n].copy_(src)
+        E = batch.n_events
+        for k, a in (("event_offset", batch.event_offset), ("layer_nhits", batch.layer_nhits),
+                     ("layer_offset", batch.layer_offset)):
+            self.host_layout[k][:a.shape[0]].copy_(torch.from_numpy(a.view(np.int32)))
+        with torch.cuda.stream(self.stream):
+            for k in self.dev:
+                self.dev[k][:n].copy_(self.host[k][:n], non_blocking=True)
+            for k in self.dev_layout:
+                m = E if k == "event_offset" else E * batch.n_layers
+                self.dev_layout[k][:m].copy_(self.host_layout[k][:m], non_blocking=True)
+            cols = dict(self.dev)
+            cols.update(self.dev_layout)
+            self.db = rt.DeviceBatch(E, n, cols)
+            rt.seed_chain(self.ctx, self.bufs, self.db, stream=self.stream)
+            self.done_event = torch.cuda.Event()
+            self.done_event.record(self.stream)
+        self.requests = reqs
+        self.batch = batch
+
+    def ready(self) -> bool:
+        return self.busy and self.done_event.query()
+
+    def finish(self, now: float) -> List[Request]:
+        """Collect the results of the finished batch (its stream is idle)."""
+        reqs, batch = self.requests, self.batch
+        flags = self.ctx.flags(stream=self.stream)
+        if flags & rt.FLAG_LAYOUT:
+            raise rt.TraxError("inconsistent event layout")
+        if flags:
+            # outputs overflowed: grow this slot's buffers and rerun synchronously
+            n_p, n_c, n_t = rt.totals(self.bufs)
+            b = self.bufs
+            self.bufs = rt.ChainBuffers(b.sc, b.max_events, b.max_hits, max(n_p, b.pair_capacity),
+                                        max(n_c, b.cand_capacity), max(n_t, b.tracklet_capacity))
+            with self.torch.cuda.stream(self.stream):
+                rt.seed_chain(self.ctx, self.bufs, self.db, stream=self.stream)
+            self.stream.synchronize()
+            if self.ctx.flags(stream=self.stream):
+                raise rt.TraxError("slot still overflows after regrowing")
+        P = len(self.bufs.sc.triplets)
+        with self.torch.cuda.stream(self.stream):
+            res = rt.fetch(self.bufs, batch.n_events, batch.n_hits)
+        offs = res["tracklet_offsets"]
+        for e, r in enumerate(reqs):
+            lo, hi = offs[e * P], offs[(e + 1) * P]
+            base = int(batch.event_offset[e])
+            r.triplets = res["orig_index"][res["tracklets"][lo:hi]].astype(np.int64) - base
+            r.done = now
+        self.requests = []
+        self.batch = None
+        return reqs
+
+
+@dataclass
+class ServeStats:
+    n_events: int
+    wall_s: float
+    events_per_sec: float
+    p50_ms: float
+    p99_ms: float
+    batches: int
+    mean_batch: float
+    max_batch: int
+    slots: int
+    budget_bytes: int
+    used_bytes: int
+    batch_sizes: List[int] = field(default_factory=list)
+
+
+class SeedServer:
+    def __init__(self, sc: ev.SeedConfig, budget_bytes: int, max_batch: int = 512, n_slots: int = 3,
+                 slot_hits: int = 1_000_000, pairs_per_hit: int = 48, device="cuda"):
+        self.sc = sc
+        self.max_batch = max_batch
+        self.budget = int(budget_bytes)
+        self.slots: List[Slot] = []
+        pair_cap = pairs_per_hit * slot_hits
+        for _ in range(n_slots):
+            s = Slot(sc, max_batch, slot_hits, pair_cap, 2 * pair_cap, pair_cap // 4, device)
+            if sum(x.nbytes() for x in self.slots) + s.nbytes() > self.budget:
+                del s
+                break
+            self.slots.append(s)
+        if not self.slots:
+            raise rt.TraxError("device-memory budget too small for one slot")
+        self.slot_hits = slot_hits
+
+    def used_bytes(self) -> int:
+        return sum(s.nbytes() for s in self.slots)
+
+    def serve(self, requests: Sequence[Request], realtime: bool = True) -> ServeStats:
+        """Serve requests in arrival order; realtime=False admits them all at t=0."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting: List[Request] = []
+        finished: List[Request] = []
+        sizes: List[int] = []
+        t0 = time.perf_counter()
+        i = 0
+        while len(finished) < len(pending):
+            now = time.perf_counter() - t0
+            while i < len(pending) and (not realtime or pending[i].arrival <= now):
+                waiting.append(pending[i])
+                i += 1
+            progressed = False
+            for s in self.slots:
+                if s.busy and s.ready():
+                    finished.extend(s.finish(time.perf_counter() - t0))
+                    progressed = True
+            for s in self.slots:
+                if not s.busy and waiting:
+                    take, hits = [], 0
+                    while waiting and len(take) < self.max_batch:
+                        h = waiting[0].batch.n_hits
+                        if hits + h > self.slot_hits:
+                            break
+                        take.append(waiting.pop(0))
+                        hits += h
+                    if not take:
+                        raise rt.TraxError("an event larger than a slot")
+                    s.launch(take)
+                    sizes.append(len(take))
+                    progressed = True
+            if not progressed:
+                time.sleep(0.0002)
+        wall = max(r.done for r in finished) - min(r.arrival for r in finished)
+        lat = np.array([(r.done - r.arrival) * 1e3 for r in finished])
+        return ServeStats(len(finished), wall, len(finished) / wall, float(np.percentile(lat, 50)),
+                          float(np.percentile(lat, 99)), len(sizes), float(np.mean(sizes)),
+                          int(max(sizes)), len(self.slots), self.budget, self.used_bytes(), sizes)
+
+
+def poisson_requests(w: ev.Workload, n: int, rate_hz: float, seed: int = ev.SEED_BASE) -> List[Request]:
+    """n events of workload w arriving as a Poisson process of the given rate."""
+    rng = np.random.default_rng([seed, 0xA77, w.config_id])
+    t = np.cumsum(rng.exponential(1.0 / rate_hz, n))
+    return [Request(k, float(t[k]), ev.make_batch(w, events=[k], seed=seed)) for k in range(n)]
