@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: no plain path, no CPU baseline, a 256 MiB "
+                         "copy kernel for counter calibration")
+    ap.add_argument("--tune", default=None, help="autotuning JSON to apply (trax_amd.autotune)")
     return ap.parse_args()
 
 
@@ -87,12 +91,20 @@ def main():
     from trax_amd import runtime
 
     torch.cuda.set_device(0)
+    if args.profile:
+        args.no_plain = args.no_cpu = True
     w = ev.config_workload(args.config, args.events)
     sc = w.seed_config
     log(f"generating {w.n_events} events ({w.name})")
     batch = ev.make_batch(w)
     log(f"{batch.n_hits} hits")
     ctx = runtime.Context(0)
+    if args.tune:
+        from trax_amd import autotune
+        with open(args.tune) as f:
+            res = autotune.TuneResult(**json.load(f))
+        autotune.apply(ctx, w, res)
+        log(f"tuned: {res.n_phi} phi x {res.n_z} z, launch {res.launch}")
     bufs, db = runtime.run_batch(ctx, sc, batch)
     torch.cuda.synchronize()
     n_pairs, n_cands, n_trk = runtime.totals(bufs)
@@ -142,12 +154,11 @@ def main():
 
     # ---- per-stage device time (eager, HIP events on the launch stream)
     st = {}
+    E = db.n_events
     st["grid_build"] = time_events(lambda: runtime.grid_build(ctx, bufs, db), 5, torch)
-    st["pairs_count"] = time_events(lambda: runtime.pairs_count(ctx, bufs, db.n_events), 5, torch)
-    st["pairs_store"] = time_events(lambda: runtime.pairs_store(ctx, bufs, db.n_events), 5, torch)
-    st["predict_count"] = time_events(lambda: runtime.predict_count(ctx, bufs, db.n_events), 5, torch)
-    st["predict_store"] = time_events(lambda: runtime.predict_store(ctx, bufs, db.n_events), 5, torch)
-    st["filter"] = time_events(lambda: runtime.triplet_filter(ctx, bufs, db.n_events), 5, torch)
+    st["pairs"] = time_events(lambda: runtime.pairs_beamspot(ctx, bufs, E), 5, torch)
+    st["predict"] = time_events(lambda: runtime.triplet_predict(ctx, bufs, E), 5, torch)
+    st["filter"] = time_events(lambda: runtime.triplet_filter(ctx, bufs, E), 5, torch)
     log("stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st.items()))
 
     # grid build roofline: 7 input columns read, 9 sorted columns written (x, y,
@@ -162,6 +173,15 @@ def main():
     scan_gbs = 8 * n_scan / (scan_ms * 1e-3) / 1e9
     log(f"grid build {grid_gbs:.0f} GB/s, scan(2^26) {scan_gbs:.0f} GB/s")
     dominant = max(st, key=st.get)
+    copy = None
+    if args.profile:
+        # plain copy of known size: the calibration kernel for FETCH_SIZE / WRITE_SIZE
+        src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        copy_ms = time_events(lambda: dst.copy_(src), 5, torch)
+        copy = {"bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4, "ms": copy_ms,
+                "gbs": 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9}
+        log(f"copy 256 MiB: {copy_ms:.3f} ms, {copy['gbs']:.0f} GB/s")
 
     # ---- plain path: one event per launch sequence, host reads between stages
     plain = None
@@ -225,6 +245,7 @@ def main():
         "roofline": {"kernel": dominant, "bound": "hbm", "ms": dom_ms, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "achieved": None, "frac": None, "traffic": None},
         "cpu_baseline": cpu,
+        "copy_calibration": copy,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

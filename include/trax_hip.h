@@ -186,6 +186,13 @@ int trax_pairs_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_even
                      const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
                      trax_seed_outputs *out, void *stream);
 
+/* Single-call form: count, resolve positions and store in one pass (no host
+ * read); writes out->pairs, out->pair_offsets and out->totals[0].  Needs only
+ * ws->work_offsets.  Graph-capture safe. */
+int trax_pairs_beamspot(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                        const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                        trax_seed_outputs *out, void *stream);
+
 /* ------------------------------------------------------------- prediction */
 /* TripletThetaPhiPredictor::run(...), TripletThetaPhiPredictor.h:43-44, in
  * two phases: count (+ prefix sum; total -> out->totals[1]) and store
@@ -197,6 +204,12 @@ int trax_predict_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_ev
                        const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
                        trax_seed_outputs *out, void *stream);
 
+/* Single-call form of the prediction (one pass; reads the pair total on the
+ * device): writes out->cands, out->cand_offsets and out->totals[1]. */
+int trax_triplet_predict(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                         const trax_hits *sorted, const uint32_t *d_grid, trax_seed_outputs *out,
+                         void *stream);
+
 /* ----------------------------------------------------------------- filter */
 /* TripletThetaPhiFilter::run(...), TripletThetaPhiFilter.h:46-48: cuts and
  * order-preserving compaction in one pass; writes t1/t2/t3, the offsets and
@@ -206,8 +219,10 @@ int trax_triplet_filter(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_e
 
 /* ------------------------------------------------------------ whole chain */
 /* The four stages back to back with no host synchronisation (the stage
- * sequence of buildTriplets, src/triplet_builder/TripletBuilder.cpp:240-263).
- * Safe to capture in a hipGraph: no allocation, no memset, no host read. */
+ * sequence of buildTriplets, src/triplet_builder/TripletBuilder.cpp:240-263):
+ * trax_grid_build, trax_pairs_beamspot, trax_triplet_predict,
+ * trax_triplet_filter.  Safe to capture in a hipGraph: no allocation, no
+ * memset, no host read. */
 int trax_seed_chain(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
                     const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,
                     const trax_workspace *ws, trax_seed_outputs *out, void *stream);

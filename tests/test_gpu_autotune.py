@@ -1,0 +1,58 @@
+"""Launch shapes change nothing but speed; the autotuner's pick stays bit-exact."""
+import numpy as np
+import pytest
+
+from helpers import rows_sorted, with_segment
+
+pytestmark = pytest.mark.gpu
+
+
+def _trk(ctx, sc, batch):
+    from trax_amd import runtime
+    bufs, _ = runtime.run_batch(ctx, sc, batch)
+    g = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+    return g
+
+
+def test_launch_shapes_do_not_change_results(ctx):
+    from trax_amd import events as ev
+    w = ev.config_workload(2, 3)
+    batch = ev.make_batch(w)
+    ref = None
+    try:
+        for waves, pb, qb, tile in [(4, 256, 256, 8192), (1, 64, 512, 4096), (8, 512, 64, 16384),
+                                    (2, 128, 128, 8192)]:
+            print(f"launch {waves} {pb} {qb} {tile}", flush=True)
+            ctx.set_launch(waves, pb, qb, tile)
+            g = _trk(ctx, w.seed_config, batch)
+            key = {k: g[k] for k in ("grid", "orig_index", "pair_offsets", "cand_offsets",
+                                     "tracklet_offsets")}
+            key["pairs"] = rows_sorted(g["pairs"])
+            key["trk"] = rows_sorted(with_segment(g["tracklets"], g["tracklet_offsets"]))
+            if ref is None:
+                ref = key
+            else:
+                for k in ref:
+                    assert np.array_equal(ref[k], key[k]), k
+    finally:
+        ctx.set_launch(4, 256, 256, 8192)
+
+
+def test_autotuned_binning_matches_oracle(ctx):
+    from oracle import bridge
+    from trax_amd import autotune
+    from trax_amd import events as ev
+    w = ev.config_workload(2, 2)
+    batch = ev.make_batch(w)
+    try:
+        res = autotune.tune(ctx, w, batch, phi_bins=(64, 128), z_bins=(32, 64), reps=1)
+        assert (res.n_phi, res.n_z) in {(64, 32), (64, 64), (128, 32), (128, 64)}
+        autotune.apply(ctx, w, res)
+        g = _trk(ctx, w.seed_config, batch)
+    finally:
+        ctx.set_launch(4, 256, 256, 8192)
+    o = bridge.run_chain(w.seed_config, batch)
+    assert np.array_equal(g["grid"], o["grid"])
+    assert np.array_equal(g["tracklet_offsets"], o["tracklet_offsets"])
+    assert np.array_equal(rows_sorted(with_segment(g["tracklets"], g["tracklet_offsets"])),
+                          rows_sorted(with_segment(o["tracklets"], o["tracklet_offsets"])))

@@ -385,7 +385,52 @@ extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint
   return TRAX_OK;
 }
 
+extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                   const trax_hits* sorted, const uint32_t* d_grid,
+                                   const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if (!ws || !ws->work_offsets || n_events > ws->max_events)
+    return fail(ctx, TRAX_ERR_INVALID, "workspace work offsets missing or too small");
+  if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals ||
+      (!out->pairs && out->pair_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
+  if (((uint64_t)ws->max_hits + 255) / 256 > trax::kStatusWords)
+    return fail(ctx, TRAX_ERR_INVALID, "too many hits for one launch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_pairs_fused(sp, sorted_view(sorted), d_grid, ws->work_offsets,
+                                               reinterpret_cast<uint2*>(out->pairs), out->pair_capacity,
+                                               out->pair_offsets, out->totals + 0, ws->max_hits,
+                                               ctx->ctrl, ctx->status, ctx->flags, s));
+  return TRAX_OK;
+}
+
 // ---------------------------------------------------------------- predictor
+extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                    const trax_hits* sorted, const uint32_t* d_grid,
+                                    trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets ||
+      !out->totals || (!out->pairs && out->pair_capacity) || (!out->cands && out->cand_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null prediction-stage buffer");
+  if ((out->pair_capacity + 255) / 256 > trax::kStatusWords)
+    return fail(ctx, TRAX_ERR_INVALID, "pair capacity too large for one launch");
+  TRAX_HIP_CHECK(ctx, trax::launch_predict_fused(sp, sorted_view(sorted), d_grid,
+                                                 reinterpret_cast<const uint2*>(out->pairs),
+                                                 out->pair_offsets, out->totals + 0, out->pair_capacity,
+                                                 reinterpret_cast<uint2*>(out->cands), out->cand_capacity,
+                                                 out->cand_offsets, out->totals + 1, ctx->ctrl,
+                                                 ctx->status, ctx->flags,
+                                                 static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
 extern "C" int trax_predict_count(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
                                   const trax_hits* sorted, const uint32_t* d_grid,
                                   const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
@@ -462,10 +507,8 @@ extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const
   int rc = trax_grid_build(ctx, cfg, in, layout, sorted, d_grid, ws, stream);
   if (rc) return rc;
   const uint32_t E = layout->n_events;
-  if ((rc = trax_pairs_count(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
-  if ((rc = trax_pairs_store(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
-  if ((rc = trax_predict_count(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
-  if ((rc = trax_predict_store(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
+  if ((rc = trax_pairs_beamspot(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
+  if ((rc = trax_triplet_predict(ctx, cfg, E, sorted, d_grid, out, stream))) return rc;
   return trax_triplet_filter(ctx, cfg, E, sorted, out, stream);
 }
 

@@ -93,12 +93,33 @@ __device__ __forceinline__ uint32_t segment_of(const uint32_t* __restrict__ offs
   return lo;
 }
 
+// Block-wide copy of a small offsets table into LDS (n <= cap), for
+// segment_of lookups without dependent global loads.  Call from all threads.
+constexpr uint32_t kLdsOffsets = 8192;
+
+// Dynamic LDS for the per-(event, layer triplet) offsets table of a launch.
+inline size_t offsets_lds_bytes(const SeedParams& sp) {
+  const uint32_t n = sp.n_events * sp.n_triplets + 1;
+  return n <= kLdsOffsets ? (size_t)n * 4u : 0u;
+}
+
+__device__ __forceinline__ const uint32_t* stage_offsets(const uint32_t* __restrict__ offs,
+                                                         uint32_t n, uint32_t* s_offs) {
+  if (n > kLdsOffsets) return offs;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_offs[i] = offs[i];
+  __syncthreads();
+  return s_offs;
+}
+
 // Status word of the look-back: hi 32 bits = (epoch << 2) | state,
 // state 1 = tile aggregate, 2 = inclusive prefix; lo 32 bits = value.
+// The word carries its own payload, so relaxed agent-scope atomics suffice:
+// they are coherent across XCDs (sc1) without the L2 write-back / invalidate
+// that release / acquire ordering adds to every access on gfx950.
 __device__ __forceinline__ void lookback_publish(unsigned long long* status, uint32_t tile,
                                                  uint32_t epoch, uint32_t state, uint32_t value) {
   unsigned long long w = ((unsigned long long)((epoch << 2) | state) << 32) | value;
-  __hip_atomic_store(&status[tile], w, __ATOMIC_RELEASE, TRAX_AGENT);
+  __hip_atomic_store(&status[tile], w, __ATOMIC_RELAXED, TRAX_AGENT);
 }
 
 // Exclusive prefix of `tile` (tile > 0), computed by one full wave.
@@ -113,7 +134,7 @@ __device__ __forceinline__ uint32_t lookback_wave(const unsigned long long* stat
     uint32_t first_p;
     while (true) {
       if (idx >= 0) {
-        unsigned long long w = __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE, TRAX_AGENT);
+        unsigned long long w = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, TRAX_AGENT);
         uint32_t hi = (uint32_t)(w >> 32);
         state = ((hi >> 2) == epoch) ? (hi & 3u) : 0u;
         value = (uint32_t)w;
@@ -133,6 +154,107 @@ __device__ __forceinline__ uint32_t lookback_wave(const unsigned long long* stat
     pred -= 64;
   }
   return excl;
+}
+
+// One tile of a single-pass compaction (256 threads = 4 waves): block-wide
+// exclusive scan of one count per thread, then the tile's global base by
+// decoupled look-back.  Returns the thread's global exclusive position; all
+// threads get the tile total and base.  Contains two block barriers.
+struct TileScan {
+  uint32_t pos, total, base;
+};
+
+__device__ __forceinline__ TileScan tile_scan_lookback(uint32_t v, uint32_t tile, uint32_t epoch,
+                                                       unsigned long long* status,
+                                                       uint32_t* s_wave, uint32_t* s_excl) {
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_inclusive_scan(v);
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t wp = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t c = s_wave[w];
+    wp += (w < (int)wave) ? c : 0u;
+    tot += c;
+  }
+  if (wave == 0) {
+    uint32_t excl = 0;
+    if (tile == 0) {
+      if (lane == 0) lookback_publish(status, tile, epoch, 2u, tot);
+    } else {
+      if (lane == 0) lookback_publish(status, tile, epoch, 1u, tot);
+      excl = lookback_wave(status, tile, epoch);
+      if (lane == 0) lookback_publish(status, tile, epoch, 2u, excl + tot);
+    }
+    if (lane == 0) *s_excl = excl;
+  }
+  __syncthreads();
+  TileScan r;
+  r.base = *s_excl;
+  r.total = tot;
+  r.pos = r.base + wp + incl - v;
+  return r;
+}
+
+// Ticket for the next tile of a persistent launch (block-uniform).
+__device__ __forceinline__ uint32_t next_tile(ScanCtrl* ctrl, uint32_t* s_tile) {
+  if (threadIdx.x == 0) *s_tile = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, TRAX_AGENT);
+  __syncthreads();
+  return *s_tile;
+}
+
+__device__ __forceinline__ uint32_t launch_epoch(ScanCtrl* ctrl) {
+  return __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, TRAX_AGENT) & 0x3FFFFFFFu;
+}
+
+// End of a persistent launch: the last block resets the tickets and advances
+// the epoch for the next launch.  Call from all threads after the tile loop.
+__device__ __forceinline__ void launch_done(ScanCtrl* ctrl, uint32_t epoch) {
+  if (threadIdx.x == 0) {
+    const uint32_t d = __hip_atomic_fetch_add(&ctrl->done, 1u, __ATOMIC_RELAXED, TRAX_AGENT);
+    if (d == gridDim.x - 1) {
+      __hip_atomic_store(&ctrl->done, 0u, __ATOMIC_RELAXED, TRAX_AGENT);
+      __hip_atomic_store(&ctrl->ticket, 0u, __ATOMIC_RELAXED, TRAX_AGENT);
+      __hip_atomic_store(&ctrl->epoch, (epoch + 1u) & 0x3FFFFFFFu, __ATOMIC_RELAXED, TRAX_AGENT);
+    }
+  }
+}
+
+// Write out[k] = pos for every segment k that starts at item `item` (empty
+// segments share their start with the next one).
+__device__ __forceinline__ void write_segment_starts(const uint32_t* offs, uint32_t K, uint32_t k,
+                                                     uint32_t item, uint32_t pos, uint32_t* out) {
+  (void)K;
+  while (offs[k] == item) {
+    out[k] = pos;
+    if (k == 0) break;
+    --k;
+  }
+}
+
+// Test hits [a, b) in index order against a (z, phi) window; loads are
+// issued four at a time so their latencies overlap.  f(idx) is called for
+// every hit inside the window, in order, and returns true to stop the walk
+// (the prediction store's slot budget).  Returns true if stopped.
+template <typename InWindow, typename F>
+__device__ __forceinline__ bool scan_range(uint32_t a, uint32_t b, const float* __restrict__ z,
+                                           const float* __restrict__ phi, InWindow in_window, F&& f) {
+  uint32_t idx = a;
+  for (; idx + 4 <= b; idx += 4) {
+    float zz[4], pp[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      zz[u] = z[idx + u];
+      pp[u] = phi[idx + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (in_window(zz[u], pp[u]) && f(idx + u)) return true;
+  }
+  for (; idx < b; ++idx)
+    if (in_window(z[idx], phi[idx]) && f(idx)) return true;
+  return false;
 }
 
 // Grid cell of a hit, GridBuilder.h:80-84 (clamped to the padding row/column).

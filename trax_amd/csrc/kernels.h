@@ -11,7 +11,7 @@ namespace trax {
 // Blocks of a persistent (ticketed) launch: 256 CUs x 8 resident blocks.
 constexpr uint64_t kPersistentBlocks = 2048;
 // Look-back status words reserved per context (tiles of a scan / filter launch).
-constexpr uint64_t kStatusWords = 1ull << 20;
+constexpr uint64_t kStatusWords = 1ull << 22;
 
 uint64_t scan_tile_elems();
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
@@ -41,6 +41,17 @@ hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_
                         const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
                         uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream);
+hipError_t launch_pairs_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                              const uint32_t* work_off, uint2* pairs, uint64_t capacity,
+                              uint32_t* pair_off, uint32_t* d_total, uint32_t max_work,
+                              ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
+                              hipStream_t stream);
+hipError_t launch_predict_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                                const uint2* pairs, const uint32_t* pair_off,
+                                const uint32_t* d_npairs, uint64_t pair_capacity, uint2* cands,
+                                uint64_t cand_capacity, uint32_t* cand_off, uint32_t* d_total,
+                                ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
+                                hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
                                  hipStream_t stream);

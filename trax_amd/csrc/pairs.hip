@@ -114,6 +114,60 @@ __global__ __launch_bounds__(256) void k_pair_plan(SeedParams sp, const uint32_t
   if (tid == 0) work_off[K] = carry;
 }
 
+// Inner hits of outer hit i that pass its window, in walk order: for each z
+// row of the window, the phi cells [phiLowSector, phiHighSector), wrapping
+// at the row end (PairGeneratorBeamspot.h:172-230).  Every index is visited
+// once (the reference's bitmap collapses its repeated wrap-around visits).
+template <typename F>
+__device__ __forceinline__ void pair_walk(const SeedParams& sp, const uint32_t* __restrict__ g1,
+                                          const PairWindow& win, const SortedHits& h, F&& f) {
+  const uint32_t nPhi = sp.n_phi;
+  auto in_window = [&](float zi, float pi) {
+    return win.zLow <= zi && zi <= win.zHigh && window_phi_ok(win.wrap, win.phiLow, win.phiHigh, pi);
+  };
+  for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
+    const uint32_t row = zs * (nPhi + 1);
+    const uint32_t zSectorStart = g1[row];
+    const uint32_t zSectorEnd = g1[row + nPhi];  // (zs+1)(nPhi+1)-1: padding cell excluded
+    const uint32_t len = zSectorEnd - zSectorStart;
+    const uint32_t A = g1[row + win.phiLowSector];
+    const uint32_t H = g1[row + win.phiHighSector];
+    const uint32_t E = win.wrap ? zSectorEnd + H - zSectorStart : H;
+    // indices j in [A, E) map to j - (j >= zSectorEnd) * len
+    scan_range(A, E < zSectorEnd ? E : zSectorEnd, h.z, h.phi, in_window, f);
+    if (E > zSectorEnd) {
+      const uint32_t m = E - len;
+      scan_range(zSectorStart, m < A ? m : A, h.z, h.phi, in_window, f);
+      if (m > zSectorEnd) scan_range(zSectorEnd, m, h.z, h.phi, in_window, f);
+    }
+  }
+}
+
+struct PairItem {
+  uint32_t k, p, i;
+  const uint32_t* g1;
+  PairWindow win;
+};
+
+// Outer hit of work item w (segment k from the LDS-staged work offsets).
+__device__ __forceinline__ PairItem pair_item(const SeedParams& sp, const SortedHits& h,
+                                              const uint32_t* __restrict__ grid,
+                                              const uint32_t* offs, uint32_t w) {
+  const uint32_t P = sp.n_triplets, K = sp.n_events * P, C = sp.cells, L = sp.n_layers;
+  PairItem it;
+  it.k = segment_of(offs, K + 1, w);
+  const uint32_t e = it.k / P;
+  it.p = it.k - e * P;
+  const uint32_t* g2 = grid + ((uint64_t)e * L + (sp.layer2[it.p] - 1)) * C;
+  const uint32_t l1 = sp.layer1[it.p] - 1;
+  it.g1 = grid + ((uint64_t)e * L + l1) * C;
+  it.i = g2[0] + (w - offs[it.k]);
+  it.win = pair_window(sp, it.p, h.x[it.i], h.y[it.i], h.z[it.i], h.phi[it.i], sp.min_radius[l1],
+                       sp.max_radius[l1]);
+  return it;
+}
+
+// Two-phase form (count -> prefix sum -> store), used by the plain method.
 template <bool STORE>
 __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
                                                const uint32_t* __restrict__ grid,
@@ -121,59 +175,87 @@ __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
                                                uint32_t* __restrict__ counts,  // count: out; store: scanned
                                                uint2* __restrict__ pairs, uint64_t capacity,
                                                uint32_t* __restrict__ flags) {
-  const uint32_t P = sp.n_triplets, K = sp.n_events * P, C = sp.cells, L = sp.n_layers;
-  const uint32_t nPhi = sp.n_phi;
+  extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
+  const uint32_t K = sp.n_events * sp.n_triplets;
+  const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
   const uint32_t W = work_off[K];
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += stride) {
-    const uint32_t k = segment_of(work_off, K + 1, w);
-    const uint32_t e = k / P, p = k - e * P;
-    const uint32_t* g2 = grid + ((uint64_t)e * L + (sp.layer2[p] - 1)) * C;
-    const uint32_t l1 = sp.layer1[p] - 1;
-    const uint32_t* g1 = grid + ((uint64_t)e * L + l1) * C;
-    const uint32_t i = g2[0] + (w - work_off[k]);
-    const PairWindow win =
-        pair_window(sp, p, h.x[i], h.y[i], h.z[i], h.phi[i], sp.min_radius[l1], sp.max_radius[l1]);
-    uint32_t n = 0;
-    uint64_t pos = STORE ? counts[w] : 0;
-    bool overflow = false;
-    auto visit = [&](uint32_t a, uint32_t b) {
-      for (uint32_t idx = a; idx < b; ++idx) {
-        const float zi = h.z[idx];
-        bool valid = win.zLow <= zi && zi <= win.zHigh;
-        valid = valid && window_phi_ok(win.wrap, win.phiLow, win.phiHigh, h.phi[idx]);
-        if (valid) {
-          if (STORE) {
-            if (pos < capacity)
-              pairs[pos] = make_uint2(idx, i);
-            else
-              overflow = true;
-            ++pos;
-          } else {
-            ++n;
-          }
-        }
-      }
-    };
-    for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
-      const uint32_t row = zs * (nPhi + 1);
-      const uint32_t zSectorStart = g1[row];
-      const uint32_t zSectorEnd = g1[row + nPhi];  // (zs+1)(nPhi+1)-1: padding cell excluded
-      const uint32_t len = zSectorEnd - zSectorStart;
-      const uint32_t A = g1[row + win.phiLowSector];
-      const uint32_t H = g1[row + win.phiHighSector];
-      const uint32_t E = win.wrap ? zSectorEnd + H - zSectorStart : H;
-      // indices j in [A, E) map to j - (j >= zSectorEnd) * len; visit each once
-      visit(A, E < zSectorEnd ? E : zSectorEnd);
-      if (E > zSectorEnd) {
-        const uint32_t m = E - len;
-        visit(zSectorStart, m < A ? m : A);
-        if (m > zSectorEnd) visit(zSectorEnd, m);
-      }
+    const PairItem it = pair_item(sp, h, grid, offs, w);
+    if (!STORE) {
+      uint32_t n = 0;
+      pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
+      counts[w] = n;
+    } else {
+      uint64_t pos = counts[w];
+      bool overflow = false;
+      pair_walk(sp, it.g1, it.win, h, [&](uint32_t idx) {
+        if (pos < capacity)
+          pairs[pos] = make_uint2(idx, it.i);
+        else
+          overflow = true;
+        ++pos;
+        return false;
+      });
+      if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
     }
-    if (!STORE) counts[w] = n;
-    if (STORE && overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
   }
+}
+
+// Single-pass form: each 256-thread block takes ticketed tiles of 256 outer
+// hits, walks each window once to count, resolves the tile's output base by
+// decoupled look-back, walks again to store, and writes the per-segment pair
+// offsets; the last tile writes the total.
+__global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h,
+                                                     const uint32_t* __restrict__ grid,
+                                                     const uint32_t* __restrict__ work_off,
+                                                     uint2* __restrict__ pairs, uint64_t capacity,
+                                                     uint32_t* __restrict__ pair_off,
+                                                     uint32_t* __restrict__ d_total, ScanCtrl* ctrl,
+                                                     unsigned long long* status,
+                                                     uint32_t* __restrict__ flags) {
+  extern __shared__ uint32_t s_offs[];
+  __shared__ uint32_t s_tile, s_excl, s_wave[4];
+  const uint32_t K = sp.n_events * sp.n_triplets;
+  const uint32_t epoch = launch_epoch(ctrl);
+  const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
+  const uint32_t W = work_off[K];
+  uint32_t ntiles = (W + 255u) / 256u;
+  if (ntiles == 0) ntiles = 1;
+  while (true) {
+    const uint32_t tile = next_tile(ctrl, &s_tile);
+    if (tile >= ntiles) break;
+    const uint32_t w = tile * 256u + threadIdx.x;
+    const bool valid = w < W;
+    PairItem it;
+    uint32_t n = 0;
+    if (valid) {
+      it = pair_item(sp, h, grid, offs, w);
+      pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
+    }
+    const TileScan ts = tile_scan_lookback(n, tile, epoch, status, s_wave, &s_excl);
+    if (valid) {
+      uint64_t pos = ts.pos;
+      bool overflow = false;
+      pair_walk(sp, it.g1, it.win, h, [&](uint32_t idx) {
+        if (pos < capacity)
+          pairs[pos] = make_uint2(idx, it.i);
+        else
+          overflow = true;
+        ++pos;
+        return false;
+      });
+      if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
+      write_segment_starts(offs, K, it.k, w, ts.pos, pair_off);
+    }
+    if (tile == ntiles - 1 && threadIdx.x == 0) {
+      const uint32_t total = ts.base + ts.total;
+      for (int64_t k = K; k >= 0 && offs[k] >= W; --k) pair_off[k] = total;
+      *d_total = total;
+    }
+    __syncthreads();  // s_tile / s_wave / s_excl are reused by the next tile
+  }
+  launch_done(ctrl, epoch);
 }
 
 // out[k] = scanned[min(idx[k], last)] for k in [0, n); total -> *d_total.
@@ -208,11 +290,24 @@ hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_
   if (blocks == 0) blocks = 1;
   if (blocks > 16384u) blocks = 16384u;
   if (store)
-    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(block), 0, stream, sp, h, grid, work_off,
+    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
                        counts, pairs, capacity, flags);
   else
-    hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(block), 0, stream, sp, h, grid, work_off,
+    hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
                        counts, pairs, capacity, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                              const uint32_t* work_off, uint2* pairs, uint64_t capacity,
+                              uint32_t* pair_off, uint32_t* d_total, uint32_t max_work,
+                              ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
+                              hipStream_t stream) {
+  uint64_t b = ((uint64_t)max_work + 255u) / 256u;
+  if (b == 0) b = 1;
+  const uint32_t blocks = (uint32_t)(b < kPersistentBlocks ? b : kPersistentBlocks);
+  hipLaunchKernelGGL(k_pairs_fused, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
+                     grid, work_off, pairs, capacity, pair_off, d_total, ctrl, status, flags);
   return hipGetLastError();
 }
 

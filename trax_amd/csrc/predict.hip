@@ -85,6 +85,94 @@ __device__ __forceinline__ PredWindow predict_window(const SeedParams& sp, uint3
   return w;
 }
 
+// Third-layer hits of one pair that pass its window, in the reference's walk
+// order (TripletThetaPhiPredictor.h:170-233, :379-456): per z row, phi cells
+// [phiLowSector, phiHighSector), wrapping at the row end; repeated visits of
+// the wrap-around walk are kept, as the reference keeps them.
+template <typename F>
+__device__ __forceinline__ void predict_walk(const SeedParams& sp, const uint32_t* __restrict__ g3,
+                                             const PredWindow& win, const SortedHits& h, F&& f) {
+  const uint32_t nPhi = sp.n_phi;
+  const float PI = TRAX_PI_F;
+  auto in_window = [&](float zi, float actPhi) {
+    return (win.zLow <= zi && zi <= win.zHigh) &&
+           ((!win.wrap && (win.phiLow <= actPhi && actPhi <= win.phiHigh)) ||
+            (win.wrap && ((win.phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= win.phiHigh))));
+  };
+  for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
+    const uint32_t row = zs * (nPhi + 1);
+    const uint32_t zSectorStart = g3[row];
+    const uint32_t zSectorEnd = g3[row + nPhi + 1];  // (zs+1)(nPhi+1): padding cell included
+    const uint32_t A = g3[row + win.phiLowSector];
+    const uint32_t H = g3[row + win.phiHighSector];
+    if (!win.wrap) {
+      if (scan_range(A, H, h.z, h.phi, in_window, f)) return;
+    } else {  // j in [A, zSectorEnd + H - zSectorStart), wrapped at zSectorEnd
+      if (scan_range(A, zSectorEnd, h.z, h.phi, in_window, f)) return;
+      if (scan_range(zSectorStart, H, h.z, h.phi, in_window, f)) return;
+    }
+  }
+}
+
+struct PredItem {
+  uint32_t k, lt;
+  const uint32_t* g3;
+  float p1x, p1y, p1z, phi1, p2x, p2y, p2z, phi2, minR, maxR;
+};
+
+__device__ __forceinline__ PredItem pred_item(const SeedParams& sp, const SortedHits& h,
+                                              const uint32_t* __restrict__ grid,
+                                              const uint2* __restrict__ pairs,
+                                              const uint32_t* offs, uint32_t q) {
+  const uint32_t P = sp.n_triplets, K = sp.n_events * P, C = sp.cells, L = sp.n_layers;
+  PredItem it;
+  it.k = segment_of(offs, K + 1, q);
+  const uint32_t e = it.k / P;
+  it.lt = it.k - e * P;
+  const uint2 pr = pairs[q];
+  const uint32_t layer = sp.layer3[it.lt] - 1;
+  it.g3 = grid + ((uint64_t)e * L + layer) * C;
+  it.p1x = h.x[pr.x], it.p1y = h.y[pr.x], it.p1z = h.z[pr.x], it.phi1 = h.phi[pr.x];
+  it.p2x = h.x[pr.y], it.p2y = h.y[pr.y], it.p2z = h.z[pr.y], it.phi2 = h.phi[pr.y];
+  it.minR = sp.min_radius[layer];
+  it.maxR = sp.max_radius[layer];
+  return it;
+}
+
+template <bool STORE>
+__device__ __forceinline__ PredWindow item_window(const SeedParams& sp, const PredItem& it) {
+  return predict_window<STORE>(sp, it.lt, it.p1x, it.p1y, it.p1z, it.phi1, it.p2x, it.p2y, it.p2z,
+                               it.phi2, it.minR, it.maxR);
+}
+
+// Store walk of pair q into slots [pos, next): the store window's hits until
+// the count's budget is used up (:394), then 0xFFFFFFFF in the unused slots.
+__device__ __forceinline__ bool predict_store_walk(const SeedParams& sp, const SortedHits& h,
+                                                   const PredItem& it, uint32_t q, uint64_t pos,
+                                                   uint64_t next, uint2* __restrict__ cands,
+                                                   uint64_t cand_capacity) {
+  bool overflow = false;
+  if (pos < next) {
+    const PredWindow win = item_window<true>(sp, it);
+    predict_walk(sp, it.g3, win, h, [&](uint32_t idx) {
+      if (pos < cand_capacity)
+        cands[pos] = make_uint2(q, idx);
+      else
+        overflow = true;
+      ++pos;
+      return pos >= next;
+    });
+  }
+  for (; pos < next; ++pos) {
+    if (pos < cand_capacity)
+      cands[pos] = make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
+    else
+      overflow = true;
+  }
+  return overflow;
+}
+
+// Two-phase form (count -> prefix sum -> store), used by the plain method.
 template <bool STORE>
 __global__ __launch_bounds__(512) void k_predict(SeedParams sp, SortedHits h,
                                                  const uint32_t* __restrict__ grid,
@@ -95,75 +183,74 @@ __global__ __launch_bounds__(512) void k_predict(SeedParams sp, SortedHits h,
                                                  uint32_t* __restrict__ counts,  // count: out; store: scanned
                                                  uint2* __restrict__ cands, uint64_t cand_capacity,
                                                  uint32_t* __restrict__ flags) {
-  const uint32_t P = sp.n_triplets, K = sp.n_events * P, C = sp.cells, L = sp.n_layers;
-  const uint32_t nPhi = sp.n_phi;
-  uint64_t nP64 = *d_npairs;
+  extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
+  const uint32_t K = sp.n_events * sp.n_triplets;
+  const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
+  const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
   const uint32_t stride = gridDim.x * blockDim.x;
-  const float PI = TRAX_PI_F;
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nP; q += stride) {
-    const uint32_t k = segment_of(pair_off, K + 1, q);
-    const uint32_t e = k / P, lt = k - e * P;
-    const uint2 pr = pairs[q];
-    const uint32_t h1 = pr.x, h2 = pr.y;
-    const uint32_t layer = sp.layer3[lt] - 1;
-    const uint32_t* g3 = grid + ((uint64_t)e * L + layer) * C;
-    const PredWindow win = predict_window<STORE>(sp, lt, h.x[h1], h.y[h1], h.z[h1], h.phi[h1],
-                                                 h.x[h2], h.y[h2], h.z[h2], h.phi[h2],
-                                                 sp.min_radius[layer], sp.max_radius[layer]);
-    uint32_t n = 0;
-    uint64_t pos = 0, next = 0;
-    if (STORE) {
-      pos = counts[q];
-      next = counts[q + 1];
-    }
-    bool overflow = false;
-    auto visit = [&](uint32_t a, uint32_t b) {
-      for (uint32_t idx = a; idx < b && (!STORE || pos < next); ++idx) {
-        const float zi = h.z[idx];
-        bool valid = win.zLow <= zi && zi <= win.zHigh;
-        const float actPhi = h.phi[idx];
-        valid = valid && ((!win.wrap && (win.phiLow <= actPhi && actPhi <= win.phiHigh)) ||
-                          (win.wrap && ((win.phiLow <= actPhi && actPhi <= PI) ||
-                                        (-PI <= actPhi && actPhi <= win.phiHigh))));
-        if (valid) {
-          if (STORE) {
-            if (pos < cand_capacity)
-              cands[pos] = make_uint2(q, idx);
-            else
-              overflow = true;
-            ++pos;
-          } else {
-            ++n;
-          }
-        }
-      }
-    };
-    for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
-      const uint32_t row = zs * (nPhi + 1);
-      const uint32_t zSectorStart = g3[row];
-      const uint32_t zSectorEnd = g3[row + nPhi + 1];  // (zs+1)(nPhi+1): padding cell included
-      const uint32_t A = g3[row + win.phiLowSector];
-      const uint32_t H = g3[row + win.phiHighSector];
-      if (!win.wrap) {
-        visit(A, H);
-      } else {  // j in [A, zSectorEnd + H - zSectorStart), wrapped at zSectorEnd
-        visit(A, zSectorEnd);
-        visit(zSectorStart, H);
-      }
-    }
+    const PredItem it = pred_item(sp, h, grid, pairs, offs, q);
     if (!STORE) {
+      uint32_t n = 0;
+      predict_walk(sp, it.g3, item_window<false>(sp, it), h, [&](uint32_t) { ++n; return false; });
       counts[q] = n;
     } else {
-      for (; pos < next; ++pos) {
-        if (pos < cand_capacity)
-          cands[pos] = make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
-        else
-          overflow = true;
-      }
-      if (overflow) atomicOr(flags, TRAX_FLAG_CAND_OVERFLOW);
+      if (predict_store_walk(sp, h, it, q, counts[q], counts[q + 1], cands, cand_capacity))
+        atomicOr(flags, TRAX_FLAG_CAND_OVERFLOW);
     }
   }
+}
+
+// Single-pass form: tiles of 256 pairs; count walk with the count window,
+// block scan + look-back for the slot base, store walk with the store window
+// into the count's slots, per-segment candidate offsets, total.
+__global__ __launch_bounds__(256) void k_predict_fused(SeedParams sp, SortedHits h,
+                                                       const uint32_t* __restrict__ grid,
+                                                       const uint2* __restrict__ pairs,
+                                                       const uint32_t* __restrict__ pair_off,
+                                                       const uint32_t* __restrict__ d_npairs,
+                                                       uint64_t pair_capacity,
+                                                       uint2* __restrict__ cands,
+                                                       uint64_t cand_capacity,
+                                                       uint32_t* __restrict__ cand_off,
+                                                       uint32_t* __restrict__ d_total, ScanCtrl* ctrl,
+                                                       unsigned long long* status,
+                                                       uint32_t* __restrict__ flags) {
+  extern __shared__ uint32_t s_offs[];
+  __shared__ uint32_t s_tile, s_excl, s_wave[4];
+  const uint32_t K = sp.n_events * sp.n_triplets;
+  const uint32_t epoch = launch_epoch(ctrl);
+  const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
+  const uint64_t nP64 = *d_npairs;
+  const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
+  uint32_t ntiles = (nP + 255u) / 256u;
+  if (ntiles == 0) ntiles = 1;
+  while (true) {
+    const uint32_t tile = next_tile(ctrl, &s_tile);
+    if (tile >= ntiles) break;
+    const uint32_t q = tile * 256u + threadIdx.x;
+    const bool valid = q < nP;
+    PredItem it;
+    uint32_t n = 0;
+    if (valid) {
+      it = pred_item(sp, h, grid, pairs, offs, q);
+      predict_walk(sp, it.g3, item_window<false>(sp, it), h, [&](uint32_t) { ++n; return false; });
+    }
+    const TileScan ts = tile_scan_lookback(n, tile, epoch, status, s_wave, &s_excl);
+    if (valid) {
+      if (predict_store_walk(sp, h, it, q, ts.pos, (uint64_t)ts.pos + n, cands, cand_capacity))
+        atomicOr(flags, TRAX_FLAG_CAND_OVERFLOW);
+      write_segment_starts(offs, K, it.k, q, ts.pos, cand_off);
+    }
+    if (tile == ntiles - 1 && threadIdx.x == 0) {
+      const uint32_t total = ts.base + ts.total;
+      for (int64_t k = K; k >= 0 && offs[k] >= nP; --k) cand_off[k] = total;
+      *d_total = total;
+    }
+    __syncthreads();
+  }
+  launch_done(ctrl, epoch);
 }
 
 hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
@@ -174,11 +261,26 @@ hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint3
   uint64_t b = (pair_capacity + block - 1) / block;
   uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 32768u ? 32768u : b));
   if (store)
-    hipLaunchKernelGGL(k_predict<true>, dim3(blocks), dim3(block), 0, stream, sp, h, grid, pairs,
+    hipLaunchKernelGGL(k_predict<true>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, pairs,
                        pair_off, d_npairs, pair_capacity, counts, cands, cand_capacity, flags);
   else
-    hipLaunchKernelGGL(k_predict<false>, dim3(blocks), dim3(block), 0, stream, sp, h, grid, pairs,
+    hipLaunchKernelGGL(k_predict<false>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, pairs,
                        pair_off, d_npairs, pair_capacity, counts, cands, cand_capacity, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_predict_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                                const uint2* pairs, const uint32_t* pair_off,
+                                const uint32_t* d_npairs, uint64_t pair_capacity, uint2* cands,
+                                uint64_t cand_capacity, uint32_t* cand_off, uint32_t* d_total,
+                                ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
+                                hipStream_t stream) {
+  uint64_t b = (pair_capacity + 255u) / 256u;
+  if (b == 0) b = 1;
+  const uint32_t blocks = (uint32_t)(b < kPersistentBlocks ? b : kPersistentBlocks);
+  hipLaunchKernelGGL(k_predict_fused, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
+                     grid, pairs, pair_off, d_npairs, pair_capacity, cands, cand_capacity, cand_off,
+                     d_total, ctrl, status, flags);
   return hipGetLastError();
 }
 

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
   const uint64_t ntiles = (n + TILE - 1) / TILE;
   // Persistent: each block takes tiles in ticket order until none are left.
   while (true) {
-    if (tid == 0) s_tile = atomicAdd(&ctrl->ticket, 1u);
+    if (tid == 0) s_tile = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, TRAX_AGENT);
     __syncthreads();
     const uint32_t tile = s_tile, epoch = s_epoch;
     if (tile >= ntiles) break;
@@ -105,11 +105,11 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
   }
   if (tid == 0) {
     const uint32_t epoch = s_epoch;
-    const uint32_t d = __hip_atomic_fetch_add(&ctrl->done, 1u, __ATOMIC_ACQ_REL, TRAX_AGENT);
+    const uint32_t d = __hip_atomic_fetch_add(&ctrl->done, 1u, __ATOMIC_RELAXED, TRAX_AGENT);
     if (d == gridDim.x - 1) {  // last block of the launch: reset for the next one
       __hip_atomic_store(&ctrl->done, 0u, __ATOMIC_RELAXED, TRAX_AGENT);
       __hip_atomic_store(&ctrl->ticket, 0u, __ATOMIC_RELAXED, TRAX_AGENT);
-      __hip_atomic_store(&ctrl->epoch, (epoch + 1u) & 0x3FFFFFFFu, __ATOMIC_RELEASE, TRAX_AGENT);
+      __hip_atomic_store(&ctrl->epoch, (epoch + 1u) & 0x3FFFFFFFu, __ATOMIC_RELAXED, TRAX_AGENT);
     }
   }
 }

@@ -95,6 +95,10 @@ SIGNATURES = {
                                    _P(OutputsC), _vp]),
     "trax_pairs_store": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp, _P(WorkspaceC),
                                    _P(OutputsC), _vp]),
+    "trax_pairs_beamspot": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
+                                      _P(WorkspaceC), _P(OutputsC), _vp]),
+    "trax_triplet_predict": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
+                                       _P(OutputsC), _vp]),
     "trax_predict_count": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
                                      _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_predict_store": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
@@ -435,6 +439,20 @@ def predict_count(ctx, bufs, n_events, stream=None):
 
 def predict_store(ctx, bufs, n_events, stream=None):
     _stage(ctx, ctx.lib.trax_predict_store, "trax_predict_store", bufs, n_events, stream=stream)
+
+
+def pairs_beamspot(ctx, bufs, n_events, stream=None):
+    """Single-pass pair stage (count, positions and store in one kernel)."""
+    _stage(ctx, ctx.lib.trax_pairs_beamspot, "trax_pairs_beamspot", bufs, n_events, stream=stream)
+
+
+def triplet_predict(ctx, bufs, n_events, stream=None):
+    """Single-pass prediction stage."""
+    cfg, hs, out = _stage_args(bufs)
+    bufs._keep = (hs, out)
+    ctx.check(ctx.lib.trax_triplet_predict(ctx.h, C.byref(cfg), n_events, C.byref(hs),
+                                           _vp(_ptr(bufs.grid)), C.byref(out), stream_ptr(stream)),
+              "trax_triplet_predict")
 
 
 def triplet_filter(ctx, bufs, n_events, stream=None):
