@@ -64,8 +64,8 @@ typedef struct {
  * MIN_PHI=-pi, MAX_PHI=pi), per-layer radius bands (GeometrySupplement,
  * index layer-1) and the layer triplets. */
 typedef struct {
-  uint32_t n_z, n_phi;
-  uint32_t n_layers; /* grid layers: detector layers 1..n_layers */
+  uint32_t n_z, n_phi; /* n_z in 1..32767, n_phi in 1..16383 */
+  uint32_t n_layers;   /* grid layers: detector layers 1..n_layers */
   float min_radius[TRAX_MAX_LAYERS];
   float max_radius[TRAX_MAX_LAYERS];
   uint32_t n_triplets;
@@ -126,8 +126,8 @@ typedef struct {
 /* Launch shape of the kernels, per context (set by the autotuner). */
 typedef struct {
   uint32_t grid_waves;    /* waves per grid-build block: 1, 2, 4 or 8 (default 4) */
-  uint32_t pairs_block;   /* threads per block, pair kernels: 64, 128, 256, 512 (default 256) */
-  uint32_t predict_block; /* threads per block, prediction kernels (default 256) */
+  uint32_t pairs_group;   /* lanes walking one outer hit's window: 16, 32, 64 (default 32) */
+  uint32_t predict_group; /* lanes walking one pair's window: 16, 32, 64 (default 16) */
   uint32_t scan_tile;     /* elements per scan tile: 4096, 8192 or 16384 (default 8192) */
 } trax_launch_config;
 

@@ -14,28 +14,49 @@ def _trk(ctx, sc, batch):
     return g
 
 
+def _plain(ctx, sc, batch):
+    """Two-phase stage calls (the plain method's kernels) on the same batch."""
+    from trax_amd import runtime
+    bufs, db = runtime.run_batch(ctx, sc, batch)
+    E = batch.n_events
+    runtime.grid_build(ctx, bufs, db)
+    runtime.pairs_count(ctx, bufs, E)
+    runtime.pairs_store(ctx, bufs, E)
+    runtime.predict_count(ctx, bufs, E)
+    runtime.predict_store(ctx, bufs, E)
+    runtime.triplet_filter(ctx, bufs, E)
+    assert ctx.flags() == 0
+    return runtime.fetch(bufs, batch.n_events, batch.n_hits)
+
+
 def test_launch_shapes_do_not_change_results(ctx):
     from trax_amd import events as ev
     w = ev.config_workload(2, 3)
     batch = ev.make_batch(w)
     ref = None
     try:
-        for waves, pb, qb, tile in [(4, 256, 256, 8192), (1, 64, 512, 4096), (8, 512, 64, 16384),
-                                    (2, 128, 128, 8192)]:
+        for waves, pb, qb, tile in [(4, 32, 16, 8192), (1, 16, 64, 4096), (8, 64, 32, 16384),
+                                    (2, 16, 16, 8192)]:
             print(f"launch {waves} {pb} {qb} {tile}", flush=True)
             ctx.set_launch(waves, pb, qb, tile)
             g = _trk(ctx, w.seed_config, batch)
+            plain = _plain(ctx, w.seed_config, batch)
             key = {k: g[k] for k in ("grid", "orig_index", "pair_offsets", "cand_offsets",
                                      "tracklet_offsets")}
             key["pairs"] = rows_sorted(g["pairs"])
             key["trk"] = rows_sorted(with_segment(g["tracklets"], g["tracklet_offsets"]))
+            for k in ("pair_offsets", "cand_offsets", "tracklet_offsets"):
+                assert np.array_equal(plain[k], g[k]), k
+            assert np.array_equal(rows_sorted(plain["pairs"]), key["pairs"])
+            assert np.array_equal(rows_sorted(with_segment(plain["tracklets"], plain["tracklet_offsets"])),
+                                  key["trk"])
             if ref is None:
                 ref = key
             else:
                 for k in ref:
                     assert np.array_equal(ref[k], key[k]), k
     finally:
-        ctx.set_launch(4, 256, 256, 8192)
+        ctx.set_launch(4, 32, 16, 8192)
 
 
 def test_autotuned_binning_matches_oracle(ctx):
@@ -50,7 +71,7 @@ def test_autotuned_binning_matches_oracle(ctx):
         autotune.apply(ctx, w, res)
         g = _trk(ctx, w.seed_config, batch)
     finally:
-        ctx.set_launch(4, 256, 256, 8192)
+        ctx.set_launch(4, 32, 16, 8192)
     o = bridge.run_chain(w.seed_config, batch)
     assert np.array_equal(g["grid"], o["grid"])
     assert np.array_equal(g["tracklet_offsets"], o["tracklet_offsets"])

@@ -21,7 +21,7 @@ struct trax_ctx {
   uint32_t* flags = nullptr;
   unsigned long long* status = nullptr;
   size_t lds_max = 65536;
-  trax_launch_config launch = {4, 256, 256, 8192};
+  trax_launch_config launch = {4, 32, 16, 8192};
 };
 
 namespace {
@@ -57,7 +57,9 @@ uint32_t bits_for(uint32_t n) {  // bits needed to hold values 0..n-1
 // Validate the configuration and build the kernel parameter block.
 int make_params(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events, trax::SeedParams* sp) {
   if (!cfg) return fail(ctx, TRAX_ERR_INVALID, "null configuration");
-  if (cfg->n_z == 0 || cfg->n_phi == 0) return fail(ctx, TRAX_ERR_INVALID, "n_z and n_phi must be > 0");
+  // window sector bounds are packed in 16-bit fields (phi up to n_phi + 1)
+  if (cfg->n_z == 0 || cfg->n_phi == 0 || cfg->n_z > 32767 || cfg->n_phi > 16383)
+    return fail(ctx, TRAX_ERR_INVALID, "n_z must be in 1..32767 and n_phi in 1..16383");
   if (cfg->n_layers == 0 || cfg->n_layers > TRAX_MAX_LAYERS)
     return fail(ctx, TRAX_ERR_INVALID, "n_layers must be in 1..16");
   if (cfg->n_triplets == 0 || cfg->n_triplets > TRAX_MAX_TRIPLETS)
@@ -204,9 +206,9 @@ extern "C" int trax_ctx_set_launch(trax_ctx* ctx, const trax_launch_config* cfg)
   if (!ctx || !cfg) return TRAX_ERR_INVALID;
   const uint32_t w = cfg->grid_waves;
   if (!(w == 1 || w == 2 || w == 4 || w == 8)) return fail(ctx, TRAX_ERR_INVALID, "grid_waves must be 1, 2, 4 or 8");
-  for (uint32_t b : {cfg->pairs_block, cfg->predict_block})
-    if (!(b == 64 || b == 128 || b == 256 || b == 512))
-      return fail(ctx, TRAX_ERR_INVALID, "block sizes must be 64, 128, 256 or 512");
+  for (uint32_t g : {cfg->pairs_group, cfg->predict_group})
+    if (!(g == 16 || g == 32 || g == 64))
+      return fail(ctx, TRAX_ERR_INVALID, "lane groups must be 16, 32 or 64");
   if (!(cfg->scan_tile == 4096 || cfg->scan_tile == 8192 || cfg->scan_tile == 16384))
     return fail(ctx, TRAX_ERR_INVALID, "scan_tile must be 4096, 8192 or 16384");
   ctx->launch = *cfg;
@@ -357,7 +359,7 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   const trax::SortedHits h = sorted_view(sorted);
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
   TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
-                                         ws->max_hits, false, ctx->flags, ctx->launch.pairs_block, s));
+                                         ws->max_hits, false, ctx->flags, ctx->launch.pairs_group, s));
   TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
                                             ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s,
                                             scan_sub(ctx)));
@@ -381,7 +383,7 @@ extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint
   TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                          ws->pair_counts, reinterpret_cast<uint2*>(out->pairs),
                                          out->pair_capacity, ws->max_hits, true, ctx->flags,
-                                         ctx->launch.pairs_block, static_cast<hipStream_t>(stream)));
+                                         ctx->launch.pairs_group, static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
 
@@ -404,7 +406,8 @@ extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, u
   TRAX_HIP_CHECK(ctx, trax::launch_pairs_fused(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                                reinterpret_cast<uint2*>(out->pairs), out->pair_capacity,
                                                out->pair_offsets, out->totals + 0, ws->max_hits,
-                                               ctx->ctrl, ctx->status, ctx->flags, s));
+                                               ctx->ctrl, ctx->status, ctx->flags,
+                                               ctx->launch.pairs_group, s));
   return TRAX_OK;
 }
 
@@ -426,7 +429,7 @@ extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, 
                                                  out->pair_offsets, out->totals + 0, out->pair_capacity,
                                                  reinterpret_cast<uint2*>(out->cands), out->cand_capacity,
                                                  out->cand_offsets, out->totals + 1, ctx->ctrl,
-                                                 ctx->status, ctx->flags,
+                                                 ctx->status, ctx->flags, ctx->launch.predict_group,
                                                  static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
@@ -448,7 +451,7 @@ extern "C" int trax_predict_count(trax_ctx* ctx, const trax_seed_config* cfg, ui
   TRAX_HIP_CHECK(ctx, trax::launch_predict(sp, sorted_view(sorted), d_grid,
                                            reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
                                            out->totals + 0, pc, ws->cand_counts, nullptr, 0, false,
-                                           ctx->flags, ctx->launch.predict_block, s));
+                                           ctx->flags, ctx->launch.predict_group, s));
   TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->cand_counts, ws->cand_counts, pc + 1, out->totals + 0, 1,
                                             ctx->ctrl, ctx->status, s, scan_sub(ctx)));
   TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->cand_counts, (uint32_t)pc, out->totals + 0,
@@ -472,7 +475,7 @@ extern "C" int trax_predict_store(trax_ctx* ctx, const trax_seed_config* cfg, ui
                                            reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
                                            out->totals + 0, pair_cap(ws, out), ws->cand_counts,
                                            reinterpret_cast<uint2*>(out->cands), out->cand_capacity, true,
-                                           ctx->flags, ctx->launch.predict_block,
+                                           ctx->flags, ctx->launch.predict_group,
                                            static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }

@@ -257,6 +257,41 @@ __device__ __forceinline__ bool scan_range(uint32_t a, uint32_t b, const float* 
   return false;
 }
 
+// Lanes of a wave split into groups of GS consecutive lanes that walk one
+// item's hit ranges together.
+template <int GS>
+struct LaneGroup {
+  uint32_t lane, g, gl;
+  uint64_t mask, below;
+  __device__ __forceinline__ LaneGroup() {
+    lane = lane_id();
+    g = lane / GS;
+    gl = lane % GS;
+    mask = GS == 64 ? ~0ull : (((1ull << GS) - 1ull) << (g * GS));
+    below = (1ull << lane) - 1ull;
+  }
+};
+
+// Group-cooperative walk of hits [a, b) (a, b uniform within the group):
+// f(idx, hit_mask, rank, count) is called once per GS-wide step by every
+// lane of the group; hit_mask marks lanes whose hit is in the window, rank is
+// the number of in-window hits on lower lanes, count the step's total.  It
+// returns true to stop the walk.  Returns true if stopped.
+template <int GS, typename InWindow, typename F>
+__device__ __forceinline__ bool group_scan_range(const LaneGroup<GS>& lg, uint32_t a, uint32_t b,
+                                                 const float* __restrict__ z,
+                                                 const float* __restrict__ phi, InWindow in_window,
+                                                 F&& f) {
+  for (uint32_t base = a; base < b; base += GS) {
+    const uint32_t idx = base + lg.gl;
+    bool p = false;
+    if (idx < b) p = in_window(z[idx], phi[idx]);
+    const uint64_t m = __ballot(p) & lg.mask;
+    if (f(idx, p, (uint32_t)__popcll(m & lg.below), (uint32_t)__popcll(m))) return true;
+  }
+  return false;
+}
+
 // Grid cell of a hit, GridBuilder.h:80-84 (clamped to the padding row/column).
 __device__ __forceinline__ uint32_t grid_cell(const SeedParams& sp, float x, float y, float z,
                                               float* phi_out) {
