@@ -126,8 +126,8 @@ typedef struct {
 /* Launch shape of the kernels, per context (set by the autotuner). */
 typedef struct {
   uint32_t grid_waves;    /* waves per grid-build block: 1, 2, 4 or 8 (default 4) */
-  uint32_t pairs_group;   /* lanes walking one outer hit's window: 16, 32, 64 (default 32) */
-  uint32_t predict_group; /* lanes walking one pair's window: 16, 32, 64 (default 16) */
+  uint32_t pairs_block;   /* threads per block, two-phase pair kernels: 64..512 (default 256) */
+  uint32_t predict_block; /* threads per block, two-phase prediction kernels (default 256) */
   uint32_t scan_tile;     /* elements per scan tile: 4096, 8192 or 16384 (default 8192) */
 } trax_launch_config;
 

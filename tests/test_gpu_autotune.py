@@ -35,8 +35,8 @@ def test_launch_shapes_do_not_change_results(ctx):
     batch = ev.make_batch(w)
     ref = None
     try:
-        for waves, pb, qb, tile in [(4, 32, 16, 8192), (1, 16, 64, 4096), (8, 64, 32, 16384),
-                                    (2, 16, 16, 8192)]:
+        for waves, pb, qb, tile in [(4, 256, 256, 8192), (1, 64, 512, 4096), (8, 512, 64, 16384),
+                                    (2, 128, 128, 8192)]:
             print(f"launch {waves} {pb} {qb} {tile}", flush=True)
             ctx.set_launch(waves, pb, qb, tile)
             g = _trk(ctx, w.seed_config, batch)
@@ -56,7 +56,7 @@ def test_launch_shapes_do_not_change_results(ctx):
                 for k in ref:
                     assert np.array_equal(ref[k], key[k]), k
     finally:
-        ctx.set_launch(4, 32, 16, 8192)
+        ctx.set_launch(4, 256, 256, 8192)
 
 
 def test_autotuned_binning_matches_oracle(ctx):
@@ -71,7 +71,7 @@ def test_autotuned_binning_matches_oracle(ctx):
         autotune.apply(ctx, w, res)
         g = _trk(ctx, w.seed_config, batch)
     finally:
-        ctx.set_launch(4, 32, 16, 8192)
+        ctx.set_launch(4, 256, 256, 8192)
     o = bridge.run_chain(w.seed_config, batch)
     assert np.array_equal(g["grid"], o["grid"])
     assert np.array_equal(g["tracklet_offsets"], o["tracklet_offsets"])

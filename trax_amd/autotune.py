@@ -1,9 +1,9 @@
 """Autotuner for the seeding chain.
 
 Searches, per workload, the grid binning (phi x z sectors) and the launch
-shape of every stage (waves per grid-build block, lanes per window walk of
-the pair and prediction kernels, scan tile size), timing each stage with
-HIP events on the launch stream.  Stages are tuned independently for a given
+shape of every stage (waves per grid-build block, threads per block of the
+two-phase pair and prediction kernels, scan tile size), timing each stage
+with HIP events on the launch stream.  Stages are tuned independently for a given
 binning (their launch shapes do not change their inputs); binnings are
 compared on the whole chain with each stage at its best shape.  Results are
 written as JSON and can be loaded into a context with `apply`.
@@ -23,7 +23,7 @@ from . import events as ev
 from . import runtime as rt
 
 WAVES = (1, 2, 4, 8)
-GROUPS = (16, 32, 64)
+BLOCKS = (64, 128, 256, 512)
 SCAN_TILES = (4096, 8192, 16384)
 PHI_BINS = (64, 128, 256)
 Z_BINS = (32, 64, 128)
@@ -59,7 +59,7 @@ def _time(fn, reps, torch) -> float:
 def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps: int = 3):
     """Best launch shape per stage at this binning; returns (launch, stage_ms, chain_ms)."""
     import torch
-    base = dict(grid_waves=4, pairs_group=32, predict_group=16, scan_tile=8192)
+    base = dict(grid_waves=4, pairs_block=256, predict_block=256, scan_tile=8192)
     ctx.set_launch(**base)
     bufs, db = rt.run_batch(ctx, sc, batch)
     E = db.n_events
@@ -79,9 +79,13 @@ def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps:
         return times
 
     sweep("grid_waves", WAVES, lambda: rt.grid_build(ctx, bufs, db), "grid_build")
-    sweep("pairs_group", GROUPS, lambda: rt.pairs_beamspot(ctx, bufs, E), "pairs")
-    sweep("predict_group", GROUPS, lambda: rt.triplet_predict(ctx, bufs, E), "predict")
-    sweep("scan_tile", SCAN_TILES, lambda: rt.pairs_count(ctx, bufs, E), "pairs_count_two_phase")
+    sweep("scan_tile", SCAN_TILES, lambda: rt.pairs_count(ctx, bufs, E), "pairs_count_scan")
+    sweep("pairs_block", BLOCKS, lambda: (rt.pairs_count(ctx, bufs, E), rt.pairs_store(ctx, bufs, E)),
+          "pairs_two_phase")
+    sweep("predict_block", BLOCKS,
+          lambda: (rt.predict_count(ctx, bufs, E), rt.predict_store(ctx, bufs, E)), "predict_two_phase")
+    stage_ms["pairs"] = _time(lambda: rt.pairs_beamspot(ctx, bufs, E), reps, torch)
+    stage_ms["predict"] = _time(lambda: rt.triplet_predict(ctx, bufs, E), reps, torch)
     ctx.set_launch(**best)
     stage_ms["filter"] = _time(lambda: rt.triplet_filter(ctx, bufs, E), reps, torch)
     chain_ms = _time(lambda: rt.seed_chain(ctx, bufs, db), reps, torch)

@@ -21,7 +21,7 @@ struct trax_ctx {
   uint32_t* flags = nullptr;
   unsigned long long* status = nullptr;
   size_t lds_max = 65536;
-  trax_launch_config launch = {4, 32, 16, 8192};
+  trax_launch_config launch = {4, 256, 256, 8192};
 };
 
 namespace {
@@ -206,9 +206,9 @@ extern "C" int trax_ctx_set_launch(trax_ctx* ctx, const trax_launch_config* cfg)
   if (!ctx || !cfg) return TRAX_ERR_INVALID;
   const uint32_t w = cfg->grid_waves;
   if (!(w == 1 || w == 2 || w == 4 || w == 8)) return fail(ctx, TRAX_ERR_INVALID, "grid_waves must be 1, 2, 4 or 8");
-  for (uint32_t g : {cfg->pairs_group, cfg->predict_group})
-    if (!(g == 16 || g == 32 || g == 64))
-      return fail(ctx, TRAX_ERR_INVALID, "lane groups must be 16, 32 or 64");
+  for (uint32_t b : {cfg->pairs_block, cfg->predict_block})
+    if (!(b == 64 || b == 128 || b == 256 || b == 512))
+      return fail(ctx, TRAX_ERR_INVALID, "block sizes must be 64, 128, 256 or 512");
   if (!(cfg->scan_tile == 4096 || cfg->scan_tile == 8192 || cfg->scan_tile == 16384))
     return fail(ctx, TRAX_ERR_INVALID, "scan_tile must be 4096, 8192 or 16384");
   ctx->launch = *cfg;
@@ -334,6 +334,7 @@ extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const
   gl.counters = nullptr;
   gl.use_lds = (uint64_t)sp.cells * 4u <= ctx->lds_max;
   gl.waves = (int)ctx->launch.grid_waves;
+  gl.lds_max = ctx->lds_max;
   if (!gl.use_lds) {
     if (!ws || !ws->grid_counters || ws->max_events < layout->n_events)
       return fail(ctx, TRAX_ERR_INVALID, "grid too large for LDS and no workspace grid counters");
@@ -359,7 +360,7 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   const trax::SortedHits h = sorted_view(sorted);
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
   TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
-                                         ws->max_hits, false, ctx->flags, ctx->launch.pairs_group, s));
+                                         ws->max_hits, false, ctx->flags, ctx->launch.pairs_block, s));
   TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
                                             ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s,
                                             scan_sub(ctx)));
@@ -383,7 +384,7 @@ extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint
   TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                          ws->pair_counts, reinterpret_cast<uint2*>(out->pairs),
                                          out->pair_capacity, ws->max_hits, true, ctx->flags,
-                                         ctx->launch.pairs_group, static_cast<hipStream_t>(stream)));
+                                         ctx->launch.pairs_block, static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
 
@@ -406,8 +407,7 @@ extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, u
   TRAX_HIP_CHECK(ctx, trax::launch_pairs_fused(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                                reinterpret_cast<uint2*>(out->pairs), out->pair_capacity,
                                                out->pair_offsets, out->totals + 0, ws->max_hits,
-                                               ctx->ctrl, ctx->status, ctx->flags,
-                                               ctx->launch.pairs_group, s));
+                                               ctx->ctrl, ctx->status, ctx->flags, s));
   return TRAX_OK;
 }
 
@@ -429,7 +429,7 @@ extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, 
                                                  out->pair_offsets, out->totals + 0, out->pair_capacity,
                                                  reinterpret_cast<uint2*>(out->cands), out->cand_capacity,
                                                  out->cand_offsets, out->totals + 1, ctx->ctrl,
-                                                 ctx->status, ctx->flags, ctx->launch.predict_group,
+                                                 ctx->status, ctx->flags,
                                                  static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
@@ -451,7 +451,7 @@ extern "C" int trax_predict_count(trax_ctx* ctx, const trax_seed_config* cfg, ui
   TRAX_HIP_CHECK(ctx, trax::launch_predict(sp, sorted_view(sorted), d_grid,
                                            reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
                                            out->totals + 0, pc, ws->cand_counts, nullptr, 0, false,
-                                           ctx->flags, ctx->launch.predict_group, s));
+                                           ctx->flags, ctx->launch.predict_block, s));
   TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->cand_counts, ws->cand_counts, pc + 1, out->totals + 0, 1,
                                             ctx->ctrl, ctx->status, s, scan_sub(ctx)));
   TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->cand_counts, (uint32_t)pc, out->totals + 0,
@@ -475,7 +475,7 @@ extern "C" int trax_predict_store(trax_ctx* ctx, const trax_seed_config* cfg, ui
                                            reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
                                            out->totals + 0, pair_cap(ws, out), ws->cand_counts,
                                            reinterpret_cast<uint2*>(out->cands), out->cand_capacity, true,
-                                           ctx->flags, ctx->launch.predict_group,
+                                           ctx->flags, ctx->launch.predict_block,
                                            static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }

@@ -83,7 +83,7 @@ __device__ __forceinline__ void block_exclusive_scan_cells(Ptr cnt, uint32_t C, 
 }
 
 template <bool LDS_CELLS, int NW>
-__global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs a) {
+__global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs a, uint32_t cache_cap) {
   constexpr uint32_t BLOCK = 64u * NW;
   extern __shared__ uint32_t s_dyn[];
   __shared__ uint32_t s_wave[NW];
@@ -100,6 +100,11 @@ __global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs 
       nh = 0;
     }
     uint32_t* cnt = LDS_CELLS ? s_dyn : (a.counters + (uint64_t)seg * C);
+    // cached path: each hit's cell and phi kept in LDS after the histogram
+    const bool cached = LDS_CELLS && nh <= cache_cap;
+    uint32_t* s_cell = s_dyn + C;
+    float* s_phi = reinterpret_cast<float*>(s_cell + cache_cap);
+    uint32_t* s_idx = s_cell + 2 * cache_cap;
     for (uint32_t c = tid; c < C; c += BLOCK) cnt[c] = 0u;
     __syncthreads();
     // histogram (gridCount, GridBuilder.h:79-85)
@@ -107,6 +112,10 @@ __global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs 
       float phi;
       uint32_t cell = grid_cell(sp, a.ix[base + i], a.iy[base + i], a.iz[base + i], &phi);
       atomicAdd(&cnt[cell], 1u);
+      if (cached) {
+        s_cell[i] = cell;
+        s_phi[i] = phi;
+      }
     }
     __syncthreads();
     block_exclusive_scan_cells<NW>(cnt, C, s_wave);
@@ -114,6 +123,32 @@ __global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs 
     uint32_t* g = a.grid + (uint64_t)seg * C;
     for (uint32_t c = tid; c < C; c += BLOCK) g[c] = base + cnt[c];
     __syncthreads();
+    if (cached) {
+      // Stable scatter without ordering the waves: every hit takes a slot of
+      // its cell (in any order), then its rank is the number of same-cell
+      // slots holding a smaller input index.  After the slot pass cnt[c] is
+      // the end of cell c, i.e. cnt[c - 1] its start.
+      for (uint32_t i = tid; i < nh; i += BLOCK) s_idx[atomicAdd(&cnt[s_cell[i]], 1u)] = i;
+      __syncthreads();
+      for (uint32_t i = tid; i < nh; i += BLOCK) {
+        const uint32_t c = s_cell[i];
+        const uint32_t lo = c == 0 ? 0u : cnt[c - 1], hi = cnt[c];
+        uint32_t rank = 0;
+        for (uint32_t t = lo; t < hi; ++t) rank += s_idx[t] < i ? 1u : 0u;
+        const uint32_t dst = base + lo + rank;
+        a.ox[dst] = a.ix[base + i];
+        a.oy[dst] = a.iy[base + i];
+        a.oz[dst] = a.iz[base + i];
+        a.olayer[dst] = a.ilayer[base + i];
+        a.odet[dst] = a.idet[base + i];
+        a.ohid[dst] = a.ihid[base + i];
+        a.oevt[dst] = a.ievt[base + i];
+        a.oorig[dst] = base + i;
+        a.ophi[dst] = s_phi[i];
+      }
+      __syncthreads();
+      continue;
+    }
     // stable scatter (gridStore, GridBuilder.h:238-253, with a stable rank)
     for (uint32_t s0 = 0; s0 < nh; s0 += BLOCK) {
       const uint32_t i = s0 + tid;
@@ -160,13 +195,13 @@ __global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs 
 }
 
 template <bool L>
-static hipError_t grid_launch_nw(int nw, uint32_t blocks, size_t lds, hipStream_t stream,
+static hipError_t grid_launch_nw(int nw, uint32_t blocks, size_t lds, uint32_t cap, hipStream_t stream,
                                  const SeedParams& sp, const GridArgs& a) {
   switch (nw) {
-    case 1: hipLaunchKernelGGL((k_grid_build<L, 1>), dim3(blocks), dim3(64), lds, stream, sp, a); break;
-    case 2: hipLaunchKernelGGL((k_grid_build<L, 2>), dim3(blocks), dim3(128), lds, stream, sp, a); break;
-    case 8: hipLaunchKernelGGL((k_grid_build<L, 8>), dim3(blocks), dim3(512), lds, stream, sp, a); break;
-    default: hipLaunchKernelGGL((k_grid_build<L, 4>), dim3(blocks), dim3(256), lds, stream, sp, a); break;
+    case 1: hipLaunchKernelGGL((k_grid_build<L, 1>), dim3(blocks), dim3(64), lds, stream, sp, a, cap); break;
+    case 2: hipLaunchKernelGGL((k_grid_build<L, 2>), dim3(blocks), dim3(128), lds, stream, sp, a, cap); break;
+    case 8: hipLaunchKernelGGL((k_grid_build<L, 8>), dim3(blocks), dim3(512), lds, stream, sp, a, cap); break;
+    default: hipLaunchKernelGGL((k_grid_build<L, 4>), dim3(blocks), dim3(256), lds, stream, sp, a, cap); break;
   }
   return hipGetLastError();
 }
@@ -186,8 +221,17 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
   const uint32_t nseg = sp.n_events * sp.n_layers;
   if (nseg == 0) return hipSuccess;
   const uint32_t blocks = nseg < 65535u ? nseg : 65535u;
-  if (gl.use_lds) return grid_launch_nw<true>(gl.waves, blocks, (size_t)sp.cells * 4u, stream, sp, a);
-  return grid_launch_nw<false>(gl.waves, blocks, 0, stream, sp, a);
+  if (gl.use_lds) {
+    // LDS: the cell counts, then (cell, phi, slot) per hit for segments of
+    // up to `cap` hits (larger segments take the recomputing path)
+    const size_t cells = (size_t)sp.cells * 4u;
+    size_t room = gl.lds_max > cells ? gl.lds_max - cells : 0;
+    uint32_t cap = (uint32_t)(room / 12u);
+    if (cap > 16384u) cap = 16384u;
+    if (cap < 256u) cap = 0;
+    return grid_launch_nw<true>(gl.waves, blocks, cells + (size_t)cap * 12u, cap, stream, sp, a);
+  }
+  return grid_launch_nw<false>(gl.waves, blocks, 0, 0, stream, sp, a);
 }
 
 hipError_t grid_build_set_lds_limit(size_t bytes) {

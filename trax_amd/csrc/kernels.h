@@ -30,7 +30,8 @@ struct GridLaunch {
   uint32_t* counters;
   uint32_t* flags;
   bool use_lds;
-  int waves;  // waves per block: 1, 2, 4 or 8
+  size_t lds_max;  // dynamic LDS a block may use
+  int waves;       // waves per block: 1, 2, 4 or 8
 };
 hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream);
 hipError_t grid_build_set_lds_limit(size_t bytes);
@@ -45,13 +46,13 @@ hipError_t launch_pairs_fused(const SeedParams& sp, const SortedHits& h, const u
                               const uint32_t* work_off, uint2* pairs, uint64_t capacity,
                               uint32_t* pair_off, uint32_t* d_total, uint32_t max_work,
                               ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                              uint32_t group, hipStream_t stream);
+                              hipStream_t stream);
 hipError_t launch_predict_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                 const uint2* pairs, const uint32_t* pair_off,
                                 const uint32_t* d_npairs, uint64_t pair_capacity, uint2* cands,
                                 uint64_t cand_capacity, uint32_t* cand_off, uint32_t* d_total,
                                 ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                                uint32_t group, hipStream_t stream);
+                                hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
                                  hipStream_t stream);

@@ -114,165 +114,98 @@ __global__ __launch_bounds__(256) void k_pair_plan(SeedParams sp, const uint32_t
   if (tid == 0) work_off[K] = carry;
 }
 
-// Windows of the 256 outer hits of a tile, one per thread, parked in LDS for
-// the lane-group walks.  Sector bounds packed lo | hi << 16 (n_z <= 32767,
-// n_phi <= 16383 by the ABI).
-struct PairTile {
-  float zlo[256], zhi[256], plo[256], phi_hi[256];
-  uint32_t zs[256];
-  uint32_t ps[256];  // phiLowSector | phiHighSector << 16 | wrap << 31
-  uint32_t g1[256];  // grid offset of the inner layer, 0xFFFFFFFF = no work item
-  uint32_t outer[256], cnt[256], pos[256];
-};
-
-// Work item w: outer hit of segment k (LDS-staged work offsets), its window.
-__device__ __forceinline__ void pair_setup(const SeedParams& sp, const SortedHits& h,
-                                           const uint32_t* __restrict__ grid, const uint32_t* offs,
-                                           uint32_t w, bool valid, PairTile& T) {
-  const uint32_t t = threadIdx.x;
-  if (!valid) {
-    T.g1[t] = 0xFFFFFFFFu;
-    return;
-  }
-  const uint32_t P = sp.n_triplets, K = sp.n_events * P, C = sp.cells, L = sp.n_layers;
-  const uint32_t k = segment_of(offs, K + 1, w);
-  const uint32_t e = k / P, p = k - e * P;
-  const uint32_t* g2 = grid + ((uint64_t)e * L + (sp.layer2[p] - 1)) * C;
-  const uint32_t l1 = sp.layer1[p] - 1;
-  const uint32_t i = g2[0] + (w - offs[k]);
-  const PairWindow win =
-      pair_window(sp, p, h.x[i], h.y[i], h.z[i], h.phi[i], sp.min_radius[l1], sp.max_radius[l1]);
-  T.g1[t] = (uint32_t)(((uint64_t)e * L + l1) * C);
-  T.outer[t] = i;
-  T.zlo[t] = win.zLow;
-  T.zhi[t] = win.zHigh;
-  T.plo[t] = win.phiLow;
-  T.phi_hi[t] = win.phiHigh;
-  T.zs[t] = (win.zLowSector < sp.n_z ? win.zLowSector : sp.n_z) | (win.zHighSector << 16);
-  T.ps[t] = win.phiLowSector | (win.phiHighSector << 16) | ((uint32_t)win.wrap << 31);
-}
-
-// Inner hits of one outer hit that pass its window, walked by a lane group:
-// per z row, the phi cells [phiLowSector, phiHighSector), wrapping at the
-// row end (PairGeneratorBeamspot.h:172-230).  Every index is visited once
-// (the reference's bitmap collapses its repeated wrap-around visits).
-template <int GS, typename F>
-__device__ __forceinline__ void pair_group_walk(const SeedParams& sp, const SortedHits& h,
-                                                const uint32_t* __restrict__ grid,
-                                                const PairTile& T, uint32_t t,
-                                                const LaneGroup<GS>& lg, F&& f) {
-  const uint32_t g1off = T.g1[t];
-  if (g1off == 0xFFFFFFFFu) return;
-  const uint32_t* g1 = grid + g1off;
+// Inner hits of outer hit i that pass its window, in walk order: for each z
+// row of the window, the phi cells [phiLowSector, phiHighSector), wrapping
+// at the row end (PairGeneratorBeamspot.h:172-230).  Every index is visited
+// once (the reference's bitmap collapses its repeated wrap-around visits).
+template <typename F>
+__device__ __forceinline__ void pair_walk(const SeedParams& sp, const uint32_t* __restrict__ g1,
+                                          const PairWindow& win, const SortedHits& h, F&& f) {
   const uint32_t nPhi = sp.n_phi;
-  const float zLow = T.zlo[t], zHigh = T.zhi[t], phiLow = T.plo[t], phiHigh = T.phi_hi[t];
-  const uint32_t zs_ = T.zs[t], ps = T.ps[t];
-  const bool wrap = (ps >> 31) != 0;
-  const uint32_t pl = ps & 0xFFFFu, ph = (ps >> 16) & 0x7FFFu;
   auto in_window = [&](float zi, float pi) {
-    return zLow <= zi && zi <= zHigh && window_phi_ok(wrap, phiLow, phiHigh, pi);
+    return win.zLow <= zi && zi <= win.zHigh && window_phi_ok(win.wrap, win.phiLow, win.phiHigh, pi);
   };
-  for (uint32_t zs = zs_ & 0xFFFFu; zs < (zs_ >> 16); ++zs) {
+  for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
     const uint32_t row = zs * (nPhi + 1);
     const uint32_t zSectorStart = g1[row];
     const uint32_t zSectorEnd = g1[row + nPhi];  // (zs+1)(nPhi+1)-1: padding cell excluded
     const uint32_t len = zSectorEnd - zSectorStart;
-    const uint32_t A = g1[row + pl];
-    const uint32_t H = g1[row + ph];
-    const uint32_t E = wrap ? zSectorEnd + H - zSectorStart : H;
+    const uint32_t A = g1[row + win.phiLowSector];
+    const uint32_t H = g1[row + win.phiHighSector];
+    const uint32_t E = win.wrap ? zSectorEnd + H - zSectorStart : H;
     // indices j in [A, E) map to j - (j >= zSectorEnd) * len
-    group_scan_range<GS>(lg, A, E < zSectorEnd ? E : zSectorEnd, h.z, h.phi, in_window, f);
+    scan_range(A, E < zSectorEnd ? E : zSectorEnd, h.z, h.phi, in_window, f);
     if (E > zSectorEnd) {
       const uint32_t m = E - len;
-      group_scan_range<GS>(lg, zSectorStart, m < A ? m : A, h.z, h.phi, in_window, f);
-      if (m > zSectorEnd) group_scan_range<GS>(lg, zSectorEnd, m, h.z, h.phi, in_window, f);
+      scan_range(zSectorStart, m < A ? m : A, h.z, h.phi, in_window, f);
+      if (m > zSectorEnd) scan_range(zSectorEnd, m, h.z, h.phi, in_window, f);
     }
   }
 }
 
-template <int GS>
-__device__ __forceinline__ void pair_count_slots(const SeedParams& sp, const SortedHits& h,
-                                                 const uint32_t* __restrict__ grid, PairTile& T,
-                                                 const LaneGroup<GS>& lg) {
-  constexpr uint32_t NG = 64 / GS;
-  const uint32_t wbase = (threadIdx.x >> 6) * 64u;
-  for (uint32_t i = 0; i < (uint32_t)GS; ++i) {
-    const uint32_t t = wbase + lg.g + NG * i;
-    uint32_t n = 0;
-    pair_group_walk<GS>(sp, h, grid, T, t, lg, [&](uint32_t, bool, uint32_t, uint32_t c) {
-      n += c;
-      return false;
-    });
-    if (lg.gl == 0) T.cnt[t] = n;
-  }
-}
+struct PairItem {
+  uint32_t k, p, i;
+  const uint32_t* g1;
+  PairWindow win;
+};
 
-template <int GS>
-__device__ __forceinline__ bool pair_store_slots(const SeedParams& sp, const SortedHits& h,
-                                                 const uint32_t* __restrict__ grid, PairTile& T,
-                                                 uint2* __restrict__ pairs, uint64_t cap,
-                                                 const LaneGroup<GS>& lg) {
-  constexpr uint32_t NG = 64 / GS;
-  const uint32_t wbase = (threadIdx.x >> 6) * 64u;
-  bool overflow = false;
-  for (uint32_t i = 0; i < (uint32_t)GS; ++i) {
-    const uint32_t t = wbase + lg.g + NG * i;
-    if (T.g1[t] == 0xFFFFFFFFu) continue;
-    const uint64_t pos = T.pos[t];
-    const uint32_t outer = T.outer[t];
-    uint32_t written = 0;
-    pair_group_walk<GS>(sp, h, grid, T, t, lg, [&](uint32_t idx, bool p, uint32_t r, uint32_t c) {
-      if (p) {
-        const uint64_t d = pos + written + r;
-        if (d < cap)
-          pairs[d] = make_uint2(idx, outer);
-        else
-          overflow = true;
-      }
-      written += c;
-      return false;
-    });
-  }
-  return overflow;
+// Outer hit of work item w (segment k from the LDS-staged work offsets).
+__device__ __forceinline__ PairItem pair_item(const SeedParams& sp, const SortedHits& h,
+                                              const uint32_t* __restrict__ grid,
+                                              const uint32_t* offs, uint32_t w) {
+  const uint32_t P = sp.n_triplets, K = sp.n_events * P, C = sp.cells, L = sp.n_layers;
+  PairItem it;
+  it.k = segment_of(offs, K + 1, w);
+  const uint32_t e = it.k / P;
+  it.p = it.k - e * P;
+  const uint32_t* g2 = grid + ((uint64_t)e * L + (sp.layer2[it.p] - 1)) * C;
+  const uint32_t l1 = sp.layer1[it.p] - 1;
+  it.g1 = grid + ((uint64_t)e * L + l1) * C;
+  it.i = g2[0] + (w - offs[it.k]);
+  it.win = pair_window(sp, it.p, h.x[it.i], h.y[it.i], h.z[it.i], h.phi[it.i], sp.min_radius[l1],
+                       sp.max_radius[l1]);
+  return it;
 }
 
 // Two-phase form (count -> prefix sum -> store), used by the plain method.
-template <int GS, bool STORE>
-__global__ __launch_bounds__(256) void k_pairs(SeedParams sp, SortedHits h,
+template <bool STORE>
+__global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
                                                const uint32_t* __restrict__ grid,
                                                const uint32_t* __restrict__ work_off,
                                                uint32_t* __restrict__ counts,  // count: out; store: scanned
                                                uint2* __restrict__ pairs, uint64_t capacity,
                                                uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
-  __shared__ PairTile T;
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
   const uint32_t W = work_off[K];
-  const LaneGroup<GS> lg;
-  bool overflow = false;
-  for (uint32_t tile = blockIdx.x; tile * 256u < W; tile += gridDim.x) {
-    const uint32_t w = tile * 256u + threadIdx.x;
-    const bool valid = w < W;
-    pair_setup(sp, h, grid, offs, w, valid, T);
-    if (STORE && valid) T.pos[threadIdx.x] = counts[w];
-    __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += stride) {
+    const PairItem it = pair_item(sp, h, grid, offs, w);
     if (!STORE) {
-      pair_count_slots<GS>(sp, h, grid, T, lg);
-      __syncthreads();
-      if (valid) counts[w] = T.cnt[threadIdx.x];
+      uint32_t n = 0;
+      pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
+      counts[w] = n;
     } else {
-      overflow |= pair_store_slots<GS>(sp, h, grid, T, pairs, capacity, lg);
+      uint64_t pos = counts[w];
+      bool overflow = false;
+      pair_walk(sp, it.g1, it.win, h, [&](uint32_t idx) {
+        if (pos < capacity)
+          pairs[pos] = make_uint2(idx, it.i);
+        else
+          overflow = true;
+        ++pos;
+        return false;
+      });
+      if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
     }
-    __syncthreads();
   }
-  if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
 }
 
-// Single-pass form: count walk, block scan + look-back for the tile's output
-// base, store walk, per-segment pair offsets and the total, for ticketed
-// tiles of 256 outer hits.
-template <int GS>
+// Single-pass form: each 256-thread block takes ticketed tiles of 256 outer
+// hits, walks each window once to count, resolves the tile's output base by
+// decoupled look-back, walks again to store, and writes the per-segment pair
+// offsets; the last tile writes the total.
 __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h,
                                                      const uint32_t* __restrict__ grid,
                                                      const uint32_t* __restrict__ work_off,
@@ -282,7 +215,6 @@ __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h
                                                      unsigned long long* status,
                                                      uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];
-  __shared__ PairTile T;
   __shared__ uint32_t s_tile, s_excl, s_wave[4];
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t epoch = launch_epoch(ctrl);
@@ -290,31 +222,39 @@ __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h
   const uint32_t W = work_off[K];
   uint32_t ntiles = (W + 255u) / 256u;
   if (ntiles == 0) ntiles = 1;
-  const LaneGroup<GS> lg;
-  bool overflow = false;
   while (true) {
     const uint32_t tile = next_tile(ctrl, &s_tile);
     if (tile >= ntiles) break;
     const uint32_t w = tile * 256u + threadIdx.x;
     const bool valid = w < W;
-    pair_setup(sp, h, grid, offs, w, valid, T);
-    __syncthreads();
-    pair_count_slots<GS>(sp, h, grid, T, lg);
-    __syncthreads();
-    const uint32_t n = valid ? T.cnt[threadIdx.x] : 0u;
+    PairItem it;
+    uint32_t n = 0;
+    if (valid) {
+      it = pair_item(sp, h, grid, offs, w);
+      pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
+    }
     const TileScan ts = tile_scan_lookback(n, tile, epoch, status, s_wave, &s_excl);
-    T.pos[threadIdx.x] = ts.pos;
-    __syncthreads();
-    overflow |= pair_store_slots<GS>(sp, h, grid, T, pairs, capacity, lg);
-    if (valid) write_segment_starts(offs, K, segment_of(offs, K + 1, w), w, ts.pos, pair_off);
+    if (valid) {
+      uint64_t pos = ts.pos;
+      bool overflow = false;
+      pair_walk(sp, it.g1, it.win, h, [&](uint32_t idx) {
+        if (pos < capacity)
+          pairs[pos] = make_uint2(idx, it.i);
+        else
+          overflow = true;
+        ++pos;
+        return false;
+      });
+      if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
+      write_segment_starts(offs, K, it.k, w, ts.pos, pair_off);
+    }
     if (tile == ntiles - 1 && threadIdx.x == 0) {
       const uint32_t total = ts.base + ts.total;
       for (int64_t k = K; k >= 0 && offs[k] >= W; --k) pair_off[k] = total;
       *d_total = total;
     }
-    __syncthreads();
+    __syncthreads();  // s_tile / s_wave / s_excl are reused by the next tile
   }
-  if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
   launch_done(ctrl, epoch);
 }
 
@@ -342,46 +282,19 @@ hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t
   return hipGetLastError();
 }
 
-template <int GS>
-static hipError_t pairs_launch(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                               const uint32_t* work_off, uint32_t* counts, uint2* pairs,
-                               uint64_t capacity, uint32_t max_work, bool store, uint32_t* flags,
-                               hipStream_t stream) {
-  uint32_t blocks = (max_work + 255u) / 256u;
-  if (blocks == 0) blocks = 1;
-  if (blocks > 65535u) blocks = 65535u;
-  if (store)
-    hipLaunchKernelGGL((k_pairs<GS, true>), dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                       grid, work_off, counts, pairs, capacity, flags);
-  else
-    hipLaunchKernelGGL((k_pairs<GS, false>), dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                       grid, work_off, counts, pairs, capacity, flags);
-  return hipGetLastError();
-}
-
-// group: lanes walking one outer hit's window together (16, 32 or 64)
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                         const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
-                        uint32_t max_work, bool store, uint32_t* flags, uint32_t group,
+                        uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream) {
-  switch (group) {
-    case 16: return pairs_launch<16>(sp, h, grid, work_off, counts, pairs, capacity, max_work, store, flags, stream);
-    case 64: return pairs_launch<64>(sp, h, grid, work_off, counts, pairs, capacity, max_work, store, flags, stream);
-    default: return pairs_launch<32>(sp, h, grid, work_off, counts, pairs, capacity, max_work, store, flags, stream);
-  }
-}
-
-template <int GS>
-static hipError_t pairs_fused_launch(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                                     const uint32_t* work_off, uint2* pairs, uint64_t capacity,
-                                     uint32_t* pair_off, uint32_t* d_total, uint32_t max_work,
-                                     ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                                     hipStream_t stream) {
-  uint64_t b = ((uint64_t)max_work + 255u) / 256u;
-  if (b == 0) b = 1;
-  const uint32_t blocks = (uint32_t)(b < kStatusWords ? b : kStatusWords);
-  hipLaunchKernelGGL((k_pairs_fused<GS>), dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                     grid, work_off, pairs, capacity, pair_off, d_total, ctrl, status, flags);
+  uint32_t blocks = (max_work + block - 1) / block;
+  if (blocks == 0) blocks = 1;
+  if (blocks > 16384u) blocks = 16384u;
+  if (store)
+    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
+                       counts, pairs, capacity, flags);
+  else
+    hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
+                       counts, pairs, capacity, flags);
   return hipGetLastError();
 }
 
@@ -389,15 +302,13 @@ hipError_t launch_pairs_fused(const SeedParams& sp, const SortedHits& h, const u
                               const uint32_t* work_off, uint2* pairs, uint64_t capacity,
                               uint32_t* pair_off, uint32_t* d_total, uint32_t max_work,
                               ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                              uint32_t group, hipStream_t stream) {
-  switch (group) {
-    case 16: return pairs_fused_launch<16>(sp, h, grid, work_off, pairs, capacity, pair_off, d_total, max_work,
-                                           ctrl, status, flags, stream);
-    case 64: return pairs_fused_launch<64>(sp, h, grid, work_off, pairs, capacity, pair_off, d_total, max_work,
-                                           ctrl, status, flags, stream);
-    default: return pairs_fused_launch<32>(sp, h, grid, work_off, pairs, capacity, pair_off, d_total, max_work,
-                                           ctrl, status, flags, stream);
-  }
+                              hipStream_t stream) {
+  uint64_t b = ((uint64_t)max_work + 255u) / 256u;
+  if (b == 0) b = 1;
+  const uint32_t blocks = (uint32_t)(b < kPersistentBlocks ? b : kPersistentBlocks);
+  hipLaunchKernelGGL(k_pairs_fused, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
+                     grid, work_off, pairs, capacity, pair_off, d_total, ctrl, status, flags);
+  return hipGetLastError();
 }
 
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,

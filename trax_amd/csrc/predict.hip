@@ -85,6 +85,35 @@ __device__ __forceinline__ PredWindow predict_window(const SeedParams& sp, uint3
   return w;
 }
 
+// Third-layer hits of one pair that pass its window, in the reference's walk
+// order (TripletThetaPhiPredictor.h:170-233, :379-456): per z row, phi cells
+// [phiLowSector, phiHighSector), wrapping at the row end; repeated visits of
+// the wrap-around walk are kept, as the reference keeps them.
+template <typename F>
+__device__ __forceinline__ void predict_walk(const SeedParams& sp, const uint32_t* __restrict__ g3,
+                                             const PredWindow& win, const SortedHits& h, F&& f) {
+  const uint32_t nPhi = sp.n_phi;
+  const float PI = TRAX_PI_F;
+  auto in_window = [&](float zi, float actPhi) {
+    return (win.zLow <= zi && zi <= win.zHigh) &&
+           ((!win.wrap && (win.phiLow <= actPhi && actPhi <= win.phiHigh)) ||
+            (win.wrap && ((win.phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= win.phiHigh))));
+  };
+  for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
+    const uint32_t row = zs * (nPhi + 1);
+    const uint32_t zSectorStart = g3[row];
+    const uint32_t zSectorEnd = g3[row + nPhi + 1];  // (zs+1)(nPhi+1): padding cell included
+    const uint32_t A = g3[row + win.phiLowSector];
+    const uint32_t H = g3[row + win.phiHighSector];
+    if (!win.wrap) {
+      if (scan_range(A, H, h.z, h.phi, in_window, f)) return;
+    } else {  // j in [A, zSectorEnd + H - zSectorStart), wrapped at zSectorEnd
+      if (scan_range(A, zSectorEnd, h.z, h.phi, in_window, f)) return;
+      if (scan_range(zSectorStart, H, h.z, h.phi, in_window, f)) return;
+    }
+  }
+}
+
 struct PredItem {
   uint32_t k, lt;
   const uint32_t* g3;
@@ -110,153 +139,42 @@ __device__ __forceinline__ PredItem pred_item(const SeedParams& sp, const Sorted
   return it;
 }
 
-// Windows of the 256 pairs of a tile, one per thread, parked in LDS so that
-// lane groups can walk them: [0] count window, [1] store window; the phi part
-// is the same for both.  Sector bounds are packed lo | hi << 16 (the ABI
-// limits n_z to 32767 and n_phi to 16383).
-struct PredTile {
-  float zlo[2][256], zhi[2][256];
-  uint32_t zs[2][256];
-  float plo[256], phi_hi[256];
-  uint32_t ps[256];  // phiLowSector | phiHighSector << 16 | wrap << 31
-  uint32_t g3[256];  // grid offset of the third layer, 0xFFFFFFFF = no pair
-  uint32_t cnt[256], pos[256];
-};
-
-__device__ __forceinline__ void pred_setup(const SeedParams& sp, const SortedHits& h,
-                                           const uint32_t* __restrict__ grid,
-                                           const uint2* __restrict__ pairs, const uint32_t* offs,
-                                           uint32_t q, bool valid, bool need_count,
-                                           bool need_store, PredTile& T) {
-  const uint32_t t = threadIdx.x;
-  if (!valid) {
-    T.g3[t] = 0xFFFFFFFFu;
-    return;
-  }
-  const PredItem it = pred_item(sp, h, grid, pairs, offs, q);
-  T.g3[t] = (uint32_t)(it.g3 - grid);
-  PredWindow w;
-  if (need_count) {
-    w = predict_window<false>(sp, it.lt, it.p1x, it.p1y, it.p1z, it.phi1, it.p2x, it.p2y, it.p2z,
-                              it.phi2, it.minR, it.maxR);
-    T.zlo[0][t] = w.zLow;
-    T.zhi[0][t] = w.zHigh;
-    T.zs[0][t] = (w.zLowSector < sp.n_z ? w.zLowSector : sp.n_z) | (w.zHighSector << 16);
-  }
-  if (need_store) {
-    w = predict_window<true>(sp, it.lt, it.p1x, it.p1y, it.p1z, it.phi1, it.p2x, it.p2y, it.p2z,
-                             it.phi2, it.minR, it.maxR);
-    T.zlo[1][t] = w.zLow;
-    T.zhi[1][t] = w.zHigh;
-    T.zs[1][t] = (w.zLowSector < sp.n_z ? w.zLowSector : sp.n_z) | (w.zHighSector << 16);
-  }
-  T.plo[t] = w.phiLow;
-  T.phi_hi[t] = w.phiHigh;
-  T.ps[t] = w.phiLowSector | (w.phiHighSector << 16) | ((uint32_t)w.wrap << 31);
+template <bool STORE>
+__device__ __forceinline__ PredWindow item_window(const SeedParams& sp, const PredItem& it) {
+  return predict_window<STORE>(sp, it.lt, it.p1x, it.p1y, it.p1z, it.phi1, it.p2x, it.p2y, it.p2z,
+                               it.phi2, it.minR, it.maxR);
 }
 
-// The reference's walk over one pair's window (TripletThetaPhiPredictor.h:
-// 170-233, :379-456), done by a lane group: per z row, phi cells
-// [phiLowSector, phiHighSector), wrapping at the row end; repeated visits of
-// the wrap-around walk are kept as the reference keeps them.  W selects the
-// count (0) or store (1) window; f as in group_scan_range.
-template <int GS, int W, typename F>
-__device__ __forceinline__ void pred_group_walk(const SeedParams& sp, const SortedHits& h,
-                                                const uint32_t* __restrict__ grid,
-                                                const PredTile& T, uint32_t t,
-                                                const LaneGroup<GS>& lg, F&& f) {
-  const uint32_t g3off = T.g3[t];
-  if (g3off == 0xFFFFFFFFu) return;
-  const uint32_t* g3 = grid + g3off;
-  const uint32_t nPhi = sp.n_phi;
-  const float zLow = T.zlo[W][t], zHigh = T.zhi[W][t];
-  const uint32_t zs_ = T.zs[W][t], ps = T.ps[t];
-  const float phiLow = T.plo[t], phiHigh = T.phi_hi[t];
-  const bool wrap = (ps >> 31) != 0;
-  const uint32_t pl = ps & 0xFFFFu, ph = (ps >> 16) & 0x7FFFu;
-  const float PI = TRAX_PI_F;
-  auto in_window = [&](float zi, float actPhi) {
-    return (zLow <= zi && zi <= zHigh) &&
-           ((!wrap && (phiLow <= actPhi && actPhi <= phiHigh)) ||
-            (wrap && ((phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= phiHigh))));
-  };
-  for (uint32_t zs = zs_ & 0xFFFFu; zs < (zs_ >> 16); ++zs) {
-    const uint32_t row = zs * (nPhi + 1);
-    const uint32_t A = g3[row + pl];
-    const uint32_t H = g3[row + ph];
-    if (!wrap) {
-      if (group_scan_range<GS>(lg, A, H, h.z, h.phi, in_window, f)) return;
-    } else {  // j in [A, zSectorEnd + H - zSectorStart), wrapped at zSectorEnd
-      const uint32_t zSectorStart = g3[row];
-      const uint32_t zSectorEnd = g3[row + nPhi + 1];  // (zs+1)(nPhi+1): padding cell included
-      if (group_scan_range<GS>(lg, A, zSectorEnd, h.z, h.phi, in_window, f)) return;
-      if (group_scan_range<GS>(lg, zSectorStart, H, h.z, h.phi, in_window, f)) return;
-    }
-  }
-}
-
-// Count every slot of this wave's 64 pairs (group g takes slots g, g + 64/GS, ...).
-template <int GS>
-__device__ __forceinline__ void pred_count_slots(const SeedParams& sp, const SortedHits& h,
-                                                 const uint32_t* __restrict__ grid, PredTile& T,
-                                                 const LaneGroup<GS>& lg) {
-  constexpr uint32_t NG = 64 / GS;
-  const uint32_t wbase = (threadIdx.x >> 6) * 64u;
-  for (uint32_t i = 0; i < (uint32_t)GS; ++i) {
-    const uint32_t t = wbase + lg.g + NG * i;
-    uint32_t n = 0;
-    pred_group_walk<GS, 0>(sp, h, grid, T, t, lg, [&](uint32_t, bool, uint32_t, uint32_t c) {
-      n += c;
-      return false;
-    });
-    if (lg.gl == 0) T.cnt[t] = n;
-  }
-}
-
-// Store the hits of every slot of this wave into [pos, pos + cnt): the store
-// window's hits in walk order until the count's budget is used (:394), then
-// 0xFFFFFFFF in the unused slots.  Returns true on capacity overflow.
-template <int GS>
-__device__ __forceinline__ bool pred_store_slots(const SeedParams& sp, const SortedHits& h,
-                                                 const uint32_t* __restrict__ grid, PredTile& T,
-                                                 uint32_t q0, uint2* __restrict__ cands,
-                                                 uint64_t cap, const LaneGroup<GS>& lg) {
-  constexpr uint32_t NG = 64 / GS;
-  const uint32_t wbase = (threadIdx.x >> 6) * 64u;
+// Store walk of pair q into slots [pos, next): the store window's hits until
+// the count's budget is used up (:394), then 0xFFFFFFFF in the unused slots.
+__device__ __forceinline__ bool predict_store_walk(const SeedParams& sp, const SortedHits& h,
+                                                   const PredItem& it, uint32_t q, uint64_t pos,
+                                                   uint64_t next, uint2* __restrict__ cands,
+                                                   uint64_t cand_capacity) {
   bool overflow = false;
-  for (uint32_t i = 0; i < (uint32_t)GS; ++i) {
-    const uint32_t t = wbase + lg.g + NG * i;
-    if (T.g3[t] == 0xFFFFFFFFu) continue;
-    const uint64_t pos = T.pos[t];
-    const uint32_t budget = T.cnt[t], q = q0 + t;
-    uint32_t written = 0;
-    if (budget > 0)
-      pred_group_walk<GS, 1>(sp, h, grid, T, t, lg, [&](uint32_t idx, bool p, uint32_t r, uint32_t c) {
-        if (p && written + r < budget) {
-          const uint64_t d = pos + written + r;
-          if (d < cap)
-            cands[d] = make_uint2(q, idx);
-          else
-            overflow = true;
-        }
-        written += c;
-        return written >= budget;
-      });
-    for (uint32_t sl = written + lg.gl; sl < budget; sl += GS) {
-      const uint64_t d = pos + sl;
-      if (d < cap)
-        cands[d] = make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
+  if (pos < next) {
+    const PredWindow win = item_window<true>(sp, it);
+    predict_walk(sp, it.g3, win, h, [&](uint32_t idx) {
+      if (pos < cand_capacity)
+        cands[pos] = make_uint2(q, idx);
       else
         overflow = true;
-    }
+      ++pos;
+      return pos >= next;
+    });
+  }
+  for (; pos < next; ++pos) {
+    if (pos < cand_capacity)
+      cands[pos] = make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
+    else
+      overflow = true;
   }
   return overflow;
 }
 
 // Two-phase form (count -> prefix sum -> store), used by the plain method.
-// Tiles of 256 pairs, block-strided.
-template <int GS, bool STORE>
-__global__ __launch_bounds__(256) void k_predict(SeedParams sp, SortedHits h,
+template <bool STORE>
+__global__ __launch_bounds__(512) void k_predict(SeedParams sp, SortedHits h,
                                                  const uint32_t* __restrict__ grid,
                                                  const uint2* __restrict__ pairs,
                                                  const uint32_t* __restrict__ pair_off,
@@ -266,38 +184,34 @@ __global__ __launch_bounds__(256) void k_predict(SeedParams sp, SortedHits h,
                                                  uint2* __restrict__ cands, uint64_t cand_capacity,
                                                  uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
-  __shared__ PredTile T;
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
-  const LaneGroup<GS> lg;
-  bool overflow = false;
-  for (uint32_t tile = blockIdx.x; tile * 256u < nP; tile += gridDim.x) {
-    const uint32_t q = tile * 256u + threadIdx.x;
-    const bool valid = q < nP;
-    pred_setup(sp, h, grid, pairs, offs, q, valid, !STORE, STORE, T);
-    if (STORE && valid) {
-      T.pos[threadIdx.x] = counts[q];
-      T.cnt[threadIdx.x] = counts[q + 1] - counts[q];
-    }
-    __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nP; q += stride) {
+    const PredItem it = pred_item(sp, h, grid, pairs, offs, q);
     if (!STORE) {
-      pred_count_slots<GS>(sp, h, grid, T, lg);
-      __syncthreads();
-      if (valid) counts[q] = T.cnt[threadIdx.x];
+      uint32_t n = 0;
+      predict_walk(sp, it.g3, item_window<false>(sp, it), h, [&](uint32_t) { ++n; return false; });
+      counts[q] = n;
     } else {
-      overflow |= pred_store_slots<GS>(sp, h, grid, T, tile * 256u, cands, cand_capacity, lg);
+      if (predict_store_walk(sp, h, it, q, counts[q], counts[q + 1], cands, cand_capacity))
+        atomicOr(flags, TRAX_FLAG_CAND_OVERFLOW);
     }
-    __syncthreads();
   }
-  if (overflow) atomicOr(flags, TRAX_FLAG_CAND_OVERFLOW);
 }
 
-// Single-pass form: count walk, block scan + look-back for the tile's slot
-// base, store walk into the count's slots, per-segment candidate offsets and
-// the total, for ticketed tiles of 256 pairs.
-template <int GS>
+// Single-pass form, ticketed tiles of 256 pairs.  One walk over the union of
+// the count and store windows (their phi parts are identical; the z bounds
+// differ by the rOrigin rounding and the zHigh margin) counts the count
+// window's hits and keeps the first kPredBuf store-window hits, in walk
+// order, in LDS.  After the block scan + look-back gives the tile's slot
+// base, each pair writes min(count, store hits) candidates from the buffer
+// (re-walking only if more than kPredBuf are needed) and fills the rest of
+// its budget with 0xFFFFFFFF, exactly as count + scan + store would.
+constexpr uint32_t kPredBuf = 8;
+
 __global__ __launch_bounds__(256) void k_predict_fused(SeedParams sp, SortedHits h,
                                                        const uint32_t* __restrict__ grid,
                                                        const uint2* __restrict__ pairs,
@@ -311,33 +225,88 @@ __global__ __launch_bounds__(256) void k_predict_fused(SeedParams sp, SortedHits
                                                        unsigned long long* status,
                                                        uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];
-  __shared__ PredTile T;
   __shared__ uint32_t s_tile, s_excl, s_wave[4];
-  const uint32_t K = sp.n_events * sp.n_triplets;
+  __shared__ uint32_t s_buf[kPredBuf][256];
+  const uint32_t K = sp.n_events * sp.n_triplets, nPhi = sp.n_phi, tid = threadIdx.x;
   const uint32_t epoch = launch_epoch(ctrl);
   const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
   uint32_t ntiles = (nP + 255u) / 256u;
   if (ntiles == 0) ntiles = 1;
-  const LaneGroup<GS> lg;
+  const float PI = TRAX_PI_F;
   bool overflow = false;
   while (true) {
     const uint32_t tile = next_tile(ctrl, &s_tile);
     if (tile >= ntiles) break;
-    const uint32_t q = tile * 256u + threadIdx.x;
+    const uint32_t q = tile * 256u + tid;
     const bool valid = q < nP;
-    pred_setup(sp, h, grid, pairs, offs, q, valid, true, true, T);
-    __syncthreads();
-    pred_count_slots<GS>(sp, h, grid, T, lg);
-    __syncthreads();
-    const uint32_t n = T.cnt[threadIdx.x];
-    const TileScan ts = tile_scan_lookback(valid ? n : 0u, tile, epoch, status, s_wave, &s_excl);
-    T.pos[threadIdx.x] = ts.pos;
-    __syncthreads();
-    overflow |= pred_store_slots<GS>(sp, h, grid, T, tile * 256u, cands, cand_capacity, lg);
-    if (valid) write_segment_starts(offs, K, segment_of(offs, K + 1, q), q, ts.pos, cand_off);
-    if (tile == ntiles - 1 && threadIdx.x == 0) {
+    PredItem it;
+    uint32_t n_count = 0, n_store = 0;
+    if (valid) {
+      it = pred_item(sp, h, grid, pairs, offs, q);
+      const PredWindow wc = item_window<false>(sp, it);
+      const PredWindow ws = item_window<true>(sp, it);
+      const bool wrap = wc.wrap;  // same for both windows
+      const float phiLow = wc.phiLow, phiHigh = wc.phiHigh;
+      const uint32_t zlo = wc.zLowSector < ws.zLowSector ? wc.zLowSector : ws.zLowSector;
+      const uint32_t zhi = wc.zHighSector > ws.zHighSector ? wc.zHighSector : ws.zHighSector;
+      for (uint32_t zs = zlo; zs < zhi; ++zs) {
+        const bool rc = zs >= wc.zLowSector && zs < wc.zHighSector;
+        const bool rs = zs >= ws.zLowSector && zs < ws.zHighSector;
+        const float czLo = wc.zLow, czHi = wc.zHigh, szLo = ws.zLow, szHi = ws.zHigh;
+        auto visit = [&](uint32_t a, uint32_t b) {
+          auto one = [&](uint32_t idx, float zi, float actPhi) {
+            const bool ph = (!wrap && (phiLow <= actPhi && actPhi <= phiHigh)) ||
+                            (wrap && ((phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= phiHigh)));
+            n_count += (rc && ph && czLo <= zi && zi <= czHi) ? 1u : 0u;
+            if (rs && ph && szLo <= zi && zi <= szHi) {
+              if (n_store < kPredBuf) s_buf[n_store][tid] = idx;
+              ++n_store;
+            }
+          };
+          uint32_t idx = a;
+          for (; idx + 4 <= b; idx += 4) {
+            float zz[4], pp[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              zz[u] = h.z[idx + u];
+              pp[u] = h.phi[idx + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) one(idx + u, zz[u], pp[u]);
+          }
+          for (; idx < b; ++idx) one(idx, h.z[idx], h.phi[idx]);
+        };
+        const uint32_t row = zs * (nPhi + 1);
+        const uint32_t A = it.g3[row + wc.phiLowSector];
+        const uint32_t H = it.g3[row + wc.phiHighSector];
+        if (!wrap) {
+          visit(A, H);
+        } else {
+          visit(A, it.g3[row + nPhi + 1]);
+          visit(it.g3[row], H);
+        }
+      }
+    }
+    const TileScan ts = tile_scan_lookback(n_count, tile, epoch, status, s_wave, &s_excl);
+    if (valid) {
+      const uint64_t pos = ts.pos;
+      const uint32_t m = n_count < n_store ? n_count : n_store;
+      if (m <= kPredBuf) {
+        for (uint32_t j = 0; j < n_count; ++j) {
+          const uint64_t d = pos + j;
+          if (d < cand_capacity)
+            cands[d] = j < m ? make_uint2(q, s_buf[j][tid]) : make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
+          else
+            overflow = true;
+        }
+      } else {
+        overflow |= predict_store_walk(sp, h, it, q, pos, pos + n_count, cands, cand_capacity);
+      }
+      write_segment_starts(offs, K, it.k, q, ts.pos, cand_off);
+    }
+    if (tile == ntiles - 1 && tid == 0) {
       const uint32_t total = ts.base + ts.total;
       for (int64_t k = K; k >= 0 && offs[k] >= nP; --k) cand_off[k] = total;
       *d_total = total;
@@ -348,53 +317,19 @@ __global__ __launch_bounds__(256) void k_predict_fused(SeedParams sp, SortedHits
   launch_done(ctrl, epoch);
 }
 
-template <int GS>
-static hipError_t predict_launch(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                                 const uint2* pairs, const uint32_t* pair_off,
-                                 const uint32_t* d_npairs, uint64_t pair_capacity, uint32_t* counts,
-                                 uint2* cands, uint64_t cand_capacity, bool store, uint32_t* flags,
-                                 hipStream_t stream) {
-  uint64_t b = (pair_capacity + 255u) / 256u;
-  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 65535u ? 65535u : b));
-  if (store)
-    hipLaunchKernelGGL((k_predict<GS, true>), dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                       grid, pairs, pair_off, d_npairs, pair_capacity, counts, cands, cand_capacity, flags);
-  else
-    hipLaunchKernelGGL((k_predict<GS, false>), dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                       grid, pairs, pair_off, d_npairs, pair_capacity, counts, cands, cand_capacity, flags);
-  return hipGetLastError();
-}
-
-// group: lanes walking one pair's window together (16, 32 or 64)
 hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                           const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                           uint64_t pair_capacity, uint32_t* counts, uint2* cands,
-                          uint64_t cand_capacity, bool store, uint32_t* flags, uint32_t group,
+                          uint64_t cand_capacity, bool store, uint32_t* flags, uint32_t block,
                           hipStream_t stream) {
-  switch (group) {
-    case 32: return predict_launch<32>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, cands,
-                                       cand_capacity, store, flags, stream);
-    case 64: return predict_launch<64>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, cands,
-                                       cand_capacity, store, flags, stream);
-    default: return predict_launch<16>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, cands,
-                                       cand_capacity, store, flags, stream);
-  }
-}
-
-template <int GS>
-static hipError_t predict_fused_launch(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                                       const uint2* pairs, const uint32_t* pair_off,
-                                       const uint32_t* d_npairs, uint64_t pair_capacity, uint2* cands,
-                                       uint64_t cand_capacity, uint32_t* cand_off, uint32_t* d_total,
-                                       ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                                       hipStream_t stream) {
-  // one tile per block; the hardware overlaps blocks waiting in look-back
-  uint64_t b = (pair_capacity + 255u) / 256u;
-  if (b == 0) b = 1;
-  const uint32_t blocks = (uint32_t)(b < kStatusWords ? b : kStatusWords);
-  hipLaunchKernelGGL((k_predict_fused<GS>), dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                     grid, pairs, pair_off, d_npairs, pair_capacity, cands, cand_capacity, cand_off,
-                     d_total, ctrl, status, flags);
+  uint64_t b = (pair_capacity + block - 1) / block;
+  uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 32768u ? 32768u : b));
+  if (store)
+    hipLaunchKernelGGL(k_predict<true>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, pairs,
+                       pair_off, d_npairs, pair_capacity, counts, cands, cand_capacity, flags);
+  else
+    hipLaunchKernelGGL(k_predict<false>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, pairs,
+                       pair_off, d_npairs, pair_capacity, counts, cands, cand_capacity, flags);
   return hipGetLastError();
 }
 
@@ -403,15 +338,14 @@ hipError_t launch_predict_fused(const SeedParams& sp, const SortedHits& h, const
                                 const uint32_t* d_npairs, uint64_t pair_capacity, uint2* cands,
                                 uint64_t cand_capacity, uint32_t* cand_off, uint32_t* d_total,
                                 ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                                uint32_t group, hipStream_t stream) {
-  switch (group) {
-    case 32: return predict_fused_launch<32>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, cands,
-                                             cand_capacity, cand_off, d_total, ctrl, status, flags, stream);
-    case 64: return predict_fused_launch<64>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, cands,
-                                             cand_capacity, cand_off, d_total, ctrl, status, flags, stream);
-    default: return predict_fused_launch<16>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, cands,
-                                             cand_capacity, cand_off, d_total, ctrl, status, flags, stream);
-  }
+                                hipStream_t stream) {
+  uint64_t b = (pair_capacity + 255u) / 256u;
+  if (b == 0) b = 1;
+  const uint32_t blocks = (uint32_t)(b < kPersistentBlocks ? b : kPersistentBlocks);
+  hipLaunchKernelGGL(k_predict_fused, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
+                     grid, pairs, pair_off, d_npairs, pair_capacity, cands, cand_capacity, cand_off,
+                     d_total, ctrl, status, flags);
+  return hipGetLastError();
 }
 
 }  // namespace trax

@@ -63,8 +63,8 @@ class WorkspaceC(C.Structure):
 
 
 class LaunchC(C.Structure):
-    _fields_ = [("grid_waves", C.c_uint32), ("pairs_group", C.c_uint32),
-                ("predict_group", C.c_uint32), ("scan_tile", C.c_uint32)]
+    _fields_ = [("grid_waves", C.c_uint32), ("pairs_block", C.c_uint32),
+                ("predict_block", C.c_uint32), ("scan_tile", C.c_uint32)]
 
 
 class OutputsC(C.Structure):
@@ -192,8 +192,8 @@ class Context:
             msg = self.lib.trax_ctx_last_error(self.h)
             raise TraxError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def set_launch(self, grid_waves=4, pairs_group=32, predict_group=16, scan_tile=8192):
-        cfg = LaunchC(grid_waves, pairs_group, predict_group, scan_tile)
+    def set_launch(self, grid_waves=4, pairs_block=256, predict_block=256, scan_tile=8192):
+        cfg = LaunchC(grid_waves, pairs_block, predict_block, scan_tile)
         self.check(self.lib.trax_ctx_set_launch(self.h, C.byref(cfg)), "trax_ctx_set_launch")
 
     def get_launch(self) -> dict:
