@@ -400,7 +400,7 @@ extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, u
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals ||
       (!out->pairs && out->pair_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
-  if (((uint64_t)ws->max_hits + 255) / 256 > trax::kStatusWords)
+  if (((uint64_t)ws->max_hits + 63) / 64 > trax::kStatusWords)  // pair wave tiles
     return fail(ctx, TRAX_ERR_INVALID, "too many hits for one launch");
   hipStream_t s = static_cast<hipStream_t>(stream);
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
@@ -422,7 +422,7 @@ extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, 
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets ||
       !out->totals || (!out->pairs && out->pair_capacity) || (!out->cands && out->cand_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null prediction-stage buffer");
-  if ((out->pair_capacity + 255) / 256 > trax::kStatusWords)
+  if ((out->pair_capacity + 63) / 64 > trax::kStatusWords)  // prediction wave tiles
     return fail(ctx, TRAX_ERR_INVALID, "pair capacity too large for one launch");
   TRAX_HIP_CHECK(ctx, trax::launch_predict_fused(sp, sorted_view(sorted), d_grid,
                                                  reinterpret_cast<const uint2*>(out->pairs),
@@ -492,7 +492,7 @@ extern "C" int trax_triplet_filter(trax_ctx* ctx, const trax_seed_config* cfg, u
       (!out->pairs && out->pair_capacity) ||
       ((!out->t1 || !out->t2 || !out->t3) && out->tracklet_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null filter-stage buffer");
-  const uint64_t tiles = (out->cand_capacity + 1023) / 1024;
+  const uint64_t tiles = (out->cand_capacity + 255) / 256;  // filter wave tiles
   if (tiles > trax::kStatusWords) return fail(ctx, TRAX_ERR_INVALID, "candidate capacity too large");
   TRAX_HIP_CHECK(ctx, trax::launch_filter(sp, sorted_view(sorted), reinterpret_cast<const uint2*>(out->pairs),
                                           reinterpret_cast<const uint2*>(out->cands), out->cand_offsets,

@@ -197,6 +197,35 @@ __device__ __forceinline__ TileScan tile_scan_lookback(uint32_t v, uint32_t tile
   return r;
 }
 
+// Wave-granular variant: each wave is a tile of 64 items with its own ticket
+// and look-back, so waves never wait for one another (irregular per-item work
+// costs the slowest lane of a wave, not of a block).
+__device__ __forceinline__ uint32_t wave_ticket(ScanCtrl* ctrl) {
+  uint32_t t = 0;
+  if (lane_id() == 0) t = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, TRAX_AGENT);
+  return __builtin_amdgcn_readfirstlane(t);
+}
+
+__device__ __forceinline__ TileScan wave_scan_lookback(uint32_t v, uint32_t tile, uint32_t epoch,
+                                                       unsigned long long* status) {
+  const uint32_t lane = lane_id();
+  const uint32_t incl = wave_inclusive_scan(v);
+  const uint32_t tot = __shfl(incl, 63, 64);
+  uint32_t excl = 0;
+  if (tile == 0) {
+    if (lane == 0) lookback_publish(status, tile, epoch, 2u, tot);
+  } else {
+    if (lane == 0) lookback_publish(status, tile, epoch, 1u, tot);
+    excl = lookback_wave(status, tile, epoch);
+    if (lane == 0) lookback_publish(status, tile, epoch, 2u, excl + tot);
+  }
+  TileScan r;
+  r.base = excl;
+  r.total = tot;
+  r.pos = excl + incl - v;
+  return r;
+}
+
 // Ticket for the next tile of a persistent launch (block-uniform).
 __device__ __forceinline__ uint32_t next_tile(ScanCtrl* ctrl, uint32_t* s_tile) {
   if (threadIdx.x == 0) *s_tile = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, TRAX_AGENT);

@@ -8,14 +8,15 @@
 // filterOffsetStore plus a racy monotonize pass derive per-(event, layer
 // triplet) offsets.
 //
-// Here one kernel does it: each 256-thread block takes the next tile of 1024
-// candidates (ticketed, so tiles resolve in order), evaluates the cuts
-// (theta ratio, phi difference, circle-fit transverse impact parameter and
-// radius; the reference's expressions in its order), ranks the accepted ones
-// with wave64 ballots, resolves the tile's output base by decoupled look-back
-// over its predecessors, and writes the (hit1, hit2, hit3) columns in
-// candidate order.  The thread that owns the first candidate of an
-// (event, layer triplet) segment writes that segment's output offset.
+// Here one kernel does it: each wave takes the next tile of 256 candidates
+// (ticketed, so tiles resolve in order), evaluates the angle cuts, queues the
+// survivors (about 7%) in LDS and runs the circle fit (TIP and radius cuts)
+// on that dense queue, ranks the accepted candidates with wave64 ballots,
+// resolves the tile's output base by decoupled look-back over its
+// predecessors, and writes the (hit1, hit2, hit3) columns in candidate order.
+// All expressions are the reference's, in its order.  The lane that owns the
+// first candidate of an (event, layer triplet) segment writes that segment's
+// output offset.
 #include "common.h"
 #include "kernels.h"
 
@@ -81,7 +82,12 @@ __device__ __forceinline__ bool filter_circle(const SeedParams& sp, uint32_t lt,
 }
 
 constexpr uint32_t kFilterSub = 4;
-constexpr uint32_t kFilterTile = 256u * kFilterSub;
+constexpr uint32_t kFilterTile = 64u * kFilterSub;  // candidates per wave tile
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
 __global__ __launch_bounds__(256) void k_filter(SeedParams sp, SortedHits h,
                                                 const uint2* __restrict__ pairs,
@@ -95,36 +101,30 @@ __global__ __launch_bounds__(256) void k_filter(SeedParams sp, SortedHits h,
                                                 uint32_t* __restrict__ d_ntrk, ScanCtrl* ctrl,
                                                 unsigned long long* status,
                                                 uint32_t* __restrict__ flags) {
-  __shared__ uint32_t s_tile, s_epoch, s_excl;
-  __shared__ uint32_t s_cnt[kFilterSub][4];
-  __shared__ uint32_t s_nq;
-  __shared__ uint16_t s_queue[kFilterTile];
-  __shared__ uint8_t s_pass[kFilterTile];
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  __shared__ uint16_t s_queue[4][kFilterTile];
+  __shared__ uint8_t s_pass[4][kFilterTile];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t P = sp.n_triplets, K = sp.n_events * P;
+  const uint32_t epoch = launch_epoch(ctrl);
   const uint32_t* offs = stage_offsets(cand_off, K + 1, s_offs);
-  if (tid == 0) s_epoch = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, TRAX_AGENT) & 0x3FFFFFFFu;
   const uint64_t nc64 = *d_ncands;
   const uint32_t nC = (uint32_t)(nc64 < cand_capacity ? nc64 : cand_capacity);
   uint32_t ntiles = (nC + kFilterTile - 1) / kFilterTile;
   if (ntiles == 0) ntiles = 1;  // tile 0 always runs so the offsets get written
-  while (true) {
-    if (tid == 0) s_tile = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, TRAX_AGENT);
-    __syncthreads();
-    const uint32_t tile = s_tile, epoch = s_epoch;
+  bool overflow = false;
+  while (true) {  // wave tiles of 256 candidates (candidate = tile * 256 + s * 64 + lane)
+    const uint32_t tile = wave_ticket(ctrl);
     if (tile >= ntiles) break;
-    // stage 1: angle cuts for the tile's 1024 candidates; survivors queued in LDS
+    // stage 1: angle cuts; survivors queued in this wave's LDS slice
     bool keep[kFilterSub];
-    uint32_t hh1[kFilterSub], hh2[kFilterSub], hh3[kFilterSub], lts[kFilterSub];
-    if (threadIdx.x == 0) s_nq = 0;
-    __syncthreads();
+    uint32_t hh1[kFilterSub], hh2[kFilterSub], hh3[kFilterSub];
+    uint32_t nq = 0;
 #pragma unroll
     for (uint32_t s = 0; s < kFilterSub; ++s) {
-      const uint32_t c = tile * kFilterTile + s * 256u + tid;
+      const uint32_t c = tile * kFilterTile + s * 64u + lane;
       keep[s] = false;
       hh1[s] = hh2[s] = hh3[s] = 0;
-      lts[s] = 0;
       if (c < nC) {
         const uint2 cd = cands[c];
         if (cd.x != TRAX_UNWRITTEN) {
@@ -132,71 +132,42 @@ __global__ __launch_bounds__(256) void k_filter(SeedParams sp, SortedHits h,
           hh1[s] = pr.x;
           hh2[s] = pr.y;
           hh3[s] = cd.y;
-          lts[s] = segment_of(offs, K + 1, c) % P;
-          keep[s] = filter_angles(sp, lts[s], h.x[pr.x], h.y[pr.x], h.z[pr.x], h.x[pr.y], h.y[pr.y],
+          const uint32_t lt = segment_of(offs, K + 1, c) % P;
+          keep[s] = filter_angles(sp, lt, h.x[pr.x], h.y[pr.x], h.z[pr.x], h.x[pr.y], h.y[pr.y],
                                   h.z[pr.y], h.x[cd.y], h.y[cd.y], h.z[cd.y]);
         }
       }
       const uint64_t m = __ballot(keep[s]);
-      uint32_t base = 0;
-      if (lane == 0 && m) base = atomicAdd(&s_nq, (uint32_t)__popcll(m));
-      base = __shfl(base, 0, 64);
-      if (keep[s]) s_queue[base + lanes_below(m)] = (s << 8) | tid;
+      if (keep[s]) s_queue[wave][nq + lanes_below(m)] = (uint16_t)(s * 64u + lane);
+      nq += (uint32_t)__popcll(m);
     }
-    __syncthreads();
+    wave_lds_sync();
     // stage 2: circle fit on the dense queue of survivors
-    const uint32_t nq = s_nq;
-    for (uint32_t j = tid; j < nq; j += 256u) {
-      const uint32_t e = s_queue[j], s = e >> 8, t = e & 255u;
-      const uint32_t c = tile * kFilterTile + s * 256u + t;
+    for (uint32_t j = lane; j < nq; j += 64u) {
+      const uint32_t e = s_queue[wave][j];
+      const uint32_t c = tile * kFilterTile + e;
       const uint2 cd = cands[c];
       const uint2 pr = pairs[cd.x];
       const uint32_t lt = segment_of(offs, K + 1, c) % P;
-      s_pass[s * 256u + t] = filter_circle(sp, lt, h.x[pr.x], h.y[pr.x], h.x[pr.y], h.y[pr.y],
-                                           h.x[cd.y], h.y[cd.y]) ? 1 : 0;
+      s_pass[wave][e] = filter_circle(sp, lt, h.x[pr.x], h.y[pr.x], h.x[pr.y], h.y[pr.y], h.x[cd.y],
+                                      h.y[cd.y]) ? 1 : 0;
     }
-    __syncthreads();
-    uint32_t rank[kFilterSub];
+    wave_lds_sync();
+    // stage 3: ordered ranks, wave look-back, store
+    uint32_t local[kFilterSub], running = 0;
 #pragma unroll
     for (uint32_t s = 0; s < kFilterSub; ++s) {
-      keep[s] = keep[s] && s_pass[s * 256u + tid];
+      keep[s] = keep[s] && s_pass[wave][s * 64u + lane];
       const uint64_t m = __ballot(keep[s]);
-      rank[s] = lanes_below(m);
-      if (lane == 0) s_cnt[s][wave] = (uint32_t)__popcll(m);
+      local[s] = running + lanes_below(m);
+      running += (uint32_t)__popcll(m);
     }
-    __syncthreads();
-    uint32_t running = 0, local[kFilterSub];
+    const TileScan ts = wave_scan_lookback(lane == 0 ? running : 0u, tile, epoch, status);
+    const uint32_t base = ts.base;
 #pragma unroll
     for (uint32_t s = 0; s < kFilterSub; ++s) {
-      uint32_t wp = 0, tot = 0;
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        const uint32_t cw = s_cnt[s][w];
-        wp += (w < (int)wave) ? cw : 0u;
-        tot += cw;
-      }
-      local[s] = running + wp + rank[s];
-      running += tot;
-    }
-    const uint32_t aggregate = running;
-    if (wave == 0) {
-      uint32_t excl = 0;
-      if (tile == 0) {
-        if (lane == 0) lookback_publish(status, tile, epoch, 2u, aggregate);
-      } else {
-        if (lane == 0) lookback_publish(status, tile, epoch, 1u, aggregate);
-        excl = lookback_wave(status, tile, epoch);
-        if (lane == 0) lookback_publish(status, tile, epoch, 2u, excl + aggregate);
-      }
-      if (lane == 0) s_excl = excl;
-    }
-    __syncthreads();
-    const uint32_t excl_tile = s_excl;
-    bool overflow = false;
-#pragma unroll
-    for (uint32_t s = 0; s < kFilterSub; ++s) {
-      const uint32_t c = tile * kFilterTile + s * 256u + tid;
-      const uint64_t pos = (uint64_t)excl_tile + local[s];
+      const uint32_t c = tile * kFilterTile + s * 64u + lane;
+      const uint64_t pos = (uint64_t)base + local[s];
       if (keep[s]) {
         if (pos < trk_capacity) {
           t1[pos] = hh1[s];
@@ -206,34 +177,19 @@ __global__ __launch_bounds__(256) void k_filter(SeedParams sp, SortedHits h,
           overflow = true;
         }
       }
-      if (c < nC) {
-        // segment starts at candidate c: their output offset is this position
-        uint32_t k = segment_of(offs, K + 1, c);
-        while (offs[k] == c) {
-          trk_off[k] = (uint32_t)pos;
-          if (k == 0) break;
-          --k;
-        }
-      }
+      // segments starting at candidate c: their output offset is this position
+      if (c < nC) write_segment_starts(offs, K, segment_of(offs, K + 1, c), c, (uint32_t)pos, trk_off);
     }
-    if (overflow) atomicOr(flags, TRAX_FLAG_TRACKLET_OVERFLOW);
-    if (tile == ntiles - 1 && tid == 0) {
-      const uint32_t total = excl_tile + aggregate;
-      // segments that start at or after the last candidate (empty tails)
+    if (tile == ntiles - 1 && lane == 0) {
+      const uint32_t total = base + running;
       for (int64_t k = K; k >= 0 && offs[k] >= nC; --k) trk_off[k] = total;
       *d_ntrk = total;
     }
-    __syncthreads();  // s_tile, s_cnt and s_excl are reused by the next tile
+    wave_lds_sync();  // the queue is reused by the next tile
   }
-  if (tid == 0) {
-    const uint32_t epoch = s_epoch;
-    uint32_t d = __hip_atomic_fetch_add(&ctrl->done, 1u, __ATOMIC_RELAXED, TRAX_AGENT);
-    if (d == gridDim.x - 1) {
-      __hip_atomic_store(&ctrl->done, 0u, __ATOMIC_RELAXED, TRAX_AGENT);
-      __hip_atomic_store(&ctrl->ticket, 0u, __ATOMIC_RELAXED, TRAX_AGENT);
-      __hip_atomic_store(&ctrl->epoch, (epoch + 1u) & 0x3FFFFFFFu, __ATOMIC_RELAXED, TRAX_AGENT);
-    }
-  }
+  if (overflow) atomicOr(flags, TRAX_FLAG_TRACKLET_OVERFLOW);
+  __syncthreads();
+  launch_done(ctrl, epoch);
 }
 
 hipError_t launch_filter(const SeedParams& sp, const SortedHits& h, const uint2* pairs,

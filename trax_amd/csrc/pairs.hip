@@ -202,10 +202,10 @@ __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
   }
 }
 
-// Single-pass form: each 256-thread block takes ticketed tiles of 256 outer
-// hits, walks each window once to count, resolves the tile's output base by
-// decoupled look-back, walks again to store, and writes the per-segment pair
-// offsets; the last tile writes the total.
+// Single-pass form: each wave takes ticketed tiles of 64 outer hits, walks
+// each window once to count, resolves the tile's output base by decoupled
+// look-back, walks again to store (the hits are in cache), and writes the
+// per-segment pair offsets; the last tile writes the total.
 __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h,
                                                      const uint32_t* __restrict__ grid,
                                                      const uint32_t* __restrict__ work_off,
@@ -215,17 +215,17 @@ __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h
                                                      unsigned long long* status,
                                                      uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];
-  __shared__ uint32_t s_tile, s_excl, s_wave[4];
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t epoch = launch_epoch(ctrl);
   const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
   const uint32_t W = work_off[K];
-  uint32_t ntiles = (W + 255u) / 256u;
+  uint32_t ntiles = (W + 63u) / 64u;
   if (ntiles == 0) ntiles = 1;
-  while (true) {
-    const uint32_t tile = next_tile(ctrl, &s_tile);
+  bool overflow = false;
+  while (true) {  // wave tiles of 64 outer hits
+    const uint32_t tile = wave_ticket(ctrl);
     if (tile >= ntiles) break;
-    const uint32_t w = tile * 256u + threadIdx.x;
+    const uint32_t w = tile * 64u + lane_id();
     const bool valid = w < W;
     PairItem it;
     uint32_t n = 0;
@@ -233,10 +233,9 @@ __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h
       it = pair_item(sp, h, grid, offs, w);
       pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
     }
-    const TileScan ts = tile_scan_lookback(n, tile, epoch, status, s_wave, &s_excl);
+    const TileScan ts = wave_scan_lookback(n, tile, epoch, status);
     if (valid) {
       uint64_t pos = ts.pos;
-      bool overflow = false;
       pair_walk(sp, it.g1, it.win, h, [&](uint32_t idx) {
         if (pos < capacity)
           pairs[pos] = make_uint2(idx, it.i);
@@ -245,16 +244,16 @@ __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h
         ++pos;
         return false;
       });
-      if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
       write_segment_starts(offs, K, it.k, w, ts.pos, pair_off);
     }
-    if (tile == ntiles - 1 && threadIdx.x == 0) {
+    if (tile == ntiles - 1 && lane_id() == 0) {
       const uint32_t total = ts.base + ts.total;
       for (int64_t k = K; k >= 0 && offs[k] >= W; --k) pair_off[k] = total;
       *d_total = total;
     }
-    __syncthreads();  // s_tile / s_wave / s_excl are reused by the next tile
   }
+  if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
+  __syncthreads();
   launch_done(ctrl, epoch);
 }
 

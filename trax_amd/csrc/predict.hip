@@ -202,7 +202,7 @@ __global__ __launch_bounds__(512) void k_predict(SeedParams sp, SortedHits h,
   }
 }
 
-// Single-pass form, ticketed tiles of 256 pairs.  One walk over the union of
+// Single-pass form, ticketed wave tiles of 64 pairs.  One walk over the union of
 // the count and store windows (their phi parts are identical; the z bounds
 // differ by the rOrigin rounding and the zHigh margin) counts the count
 // window's hits and keeps the first kPredBuf store-window hits, in walk
@@ -225,21 +225,20 @@ __global__ __launch_bounds__(256) void k_predict_fused(SeedParams sp, SortedHits
                                                        unsigned long long* status,
                                                        uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];
-  __shared__ uint32_t s_tile, s_excl, s_wave[4];
   __shared__ uint32_t s_buf[kPredBuf][256];
   const uint32_t K = sp.n_events * sp.n_triplets, nPhi = sp.n_phi, tid = threadIdx.x;
   const uint32_t epoch = launch_epoch(ctrl);
   const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
-  uint32_t ntiles = (nP + 255u) / 256u;
+  uint32_t ntiles = (nP + 63u) / 64u;
   if (ntiles == 0) ntiles = 1;
   const float PI = TRAX_PI_F;
   bool overflow = false;
-  while (true) {
-    const uint32_t tile = next_tile(ctrl, &s_tile);
+  while (true) {  // wave tiles of 64 pairs
+    const uint32_t tile = wave_ticket(ctrl);
     if (tile >= ntiles) break;
-    const uint32_t q = tile * 256u + tid;
+    const uint32_t q = tile * 64u + lane_id();
     const bool valid = q < nP;
     PredItem it;
     uint32_t n_count = 0, n_store = 0;
@@ -289,7 +288,7 @@ __global__ __launch_bounds__(256) void k_predict_fused(SeedParams sp, SortedHits
         }
       }
     }
-    const TileScan ts = tile_scan_lookback(n_count, tile, epoch, status, s_wave, &s_excl);
+    const TileScan ts = wave_scan_lookback(n_count, tile, epoch, status);
     if (valid) {
       const uint64_t pos = ts.pos;
       const uint32_t m = n_count < n_store ? n_count : n_store;
@@ -306,14 +305,14 @@ __global__ __launch_bounds__(256) void k_predict_fused(SeedParams sp, SortedHits
       }
       write_segment_starts(offs, K, it.k, q, ts.pos, cand_off);
     }
-    if (tile == ntiles - 1 && tid == 0) {
+    if (tile == ntiles - 1 && lane_id() == 0) {
       const uint32_t total = ts.base + ts.total;
       for (int64_t k = K; k >= 0 && offs[k] >= nP; --k) cand_off[k] = total;
       *d_total = total;
     }
-    __syncthreads();
   }
   if (overflow) atomicOr(flags, TRAX_FLAG_CAND_OVERFLOW);
+  __syncthreads();
   launch_done(ctrl, epoch);
 }
 
