@@ -62,6 +62,43 @@ def time_events(fn, reps: int, torch):
     return a.elapsed_time(b) / reps
 
 
+# Algorithmic bytes per launch of each chain kernel: what it must read and
+# write at least once (inputs read once, outputs written once), from the
+# batch's counts.  c: hits, pairs, cands (candidate slots), triplets, cells.
+ALGO_BYTES = {
+    # 7 input columns read, 9 sorted columns written, one uint32 per cell
+    "grid_build": lambda c: 64 * c["hits"] + 4 * c["cells"],
+    # x, y, z, phi of every hit and the cells read, one uint2 per pair written
+    "pairs": lambda c: 16 * c["hits"] + 4 * c["cells"] + 8 * c["pairs"],
+    # pair read + count written per pair, z and phi of every hit and the cells
+    # read, one stash slot per candidate slot written
+    "predict_walk": lambda c: 16 * c["hits"] + 4 * c["cells"] + 12 * c["pairs"] + 4 * c["cands"],
+    # scanned count per pair and stash slot per candidate read, uint2 written
+    "predict_place": lambda c: 4 * c["pairs"] + 12 * c["cands"],
+    # candidate and pair records and x, y, z of every hit read, mask words
+    # and their counts written
+    "filter_mask": lambda c: 8 * c["cands"] + 8 * c["pairs"] + 12 * c["hits"] + 12 * c["cands"] // 64,
+    # mask words + scanned counts read; per triplet a candidate and a pair
+    # read and three hit indexes written
+    "filter_store": lambda c: 12 * c["cands"] // 64 + 28 * c["triplets"],
+}
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary covers this workload and kernel."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
+            if parts.get("kernel") == kernel and "hbm_bytes" in parts:
+                return float(parts["hbm_bytes"])
+    return None
+
+
 def cpu_baseline(sc, batch, threads: int):
     """The oracle chain over every event of the batch, a thread per event
     (ctypes releases the GIL), timed on the host cores."""
@@ -161,24 +198,39 @@ def main():
     st["filter"] = time_events(lambda: runtime.triplet_filter(ctx, bufs, E), 5, torch)
     log("stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st.items()))
 
-    # grid build roofline: 7 input columns read, 9 sorted columns written (x, y,
-    # z, layer, detId, hitId, evt, orig_index, phi), one uint32 per cell written
-    C = sc.cells_per_layer
-    grid_bytes = batch.n_hits * (7 * 4 + 9 * 4) + batch.n_events * sc.n_layers * C * 4
-    grid_gbs = grid_bytes / (st["grid_build"] * 1e-3) / 1e9
+    # ---- per-kernel device time: the library brackets every launch of the
+    # selected kernel with HIP events on its stream, over eager chain steps
+    counts = {"hits": batch.n_hits, "pairs": n_pairs, "cands": n_cands, "triplets": n_trk,
+              "cells": batch.n_events * sc.n_layers * sc.cells_per_layer}
+    kern = {}
+    reps = max(3, min(args.steps, 10))
+    for name in ("grid_build", "pairs", "predict_walk", "predict_place", "filter_mask",
+                 "filter_store", "scan"):
+        ms_launch, n = ctx.time_kernel(name, lambda: runtime.seed_chain(ctx, bufs, db), reps)
+        kern[name] = {"ms": ms_launch, "launches_per_step": n / reps}
+        if name in ALGO_BYTES:
+            b = ALGO_BYTES[name](counts)
+            kern[name].update(bytes=b, gbs=b / (ms_launch * 1e-3) / 1e9)
+    torch.cuda.synchronize()
+    if ctx.flags():
+        raise SystemExit("overflow during the per-kernel timing")
+    log("kernels ms/launch: " + ", ".join(f"{k} {v['ms']:.3f}" for k, v in kern.items()))
+    dominant = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["launches_per_step"])
+
     # scan roofline on a large array (4 B read + 4 B write per element)
     n_scan = 1 << 26
     big = torch.zeros(n_scan, dtype=torch.int32, device="cuda")
-    scan_ms = time_events(lambda: runtime.prefix_sum(ctx, big), 10, torch)
+    scan_ms, _ = ctx.time_kernel("scan", lambda: runtime.prefix_sum(ctx, big), 10)
     scan_gbs = 8 * n_scan / (scan_ms * 1e-3) / 1e9
-    log(f"grid build {grid_gbs:.0f} GB/s, scan(2^26) {scan_gbs:.0f} GB/s")
-    dominant = max(st, key=st.get)
+    grid_gbs = kern["grid_build"]["gbs"]
+    log(f"grid build {grid_gbs:.0f} GB/s, scan(2^26) {scan_gbs:.0f} GB/s, dominant {dominant}")
     copy = None
     if args.profile:
-        # plain copy of known size: the calibration kernel for FETCH_SIZE / WRITE_SIZE
+        # elementwise kernel of known traffic (256 MiB read, 256 MiB written):
+        # the calibration kernel for FETCH_SIZE / WRITE_SIZE
         src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
         dst = torch.empty_like(src)
-        copy_ms = time_events(lambda: dst.copy_(src), 5, torch)
+        copy_ms = time_events(lambda: torch.neg(src, out=dst), 5, torch)
         copy = {"bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4, "ms": copy_ms,
                 "gbs": 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9}
         log(f"copy 256 MiB: {copy_ms:.3f} ms, {copy['gbs']:.0f} GB/s")
@@ -216,7 +268,8 @@ def main():
                "sample": f"{batch.n_events} events of the same workload, C oracle chain, "
                          f"{threads} threads, {cdt:.1f} s wall"}
 
-    dom_ms = st[dominant]
+    dom = kern[dominant]
+    traffic = measured_traffic(w.name, dominant)
     out = {
         "metric": "events/sec (batched seeding chain, 1 GPU)",
         "value": ev_s,
@@ -238,12 +291,14 @@ def main():
         "plain": plain,
         "batched_over_plain": (ev_s / plain["events_per_sec"]) if plain else None,
         "stages_ms": st,
+        "kernels": kern,
         "grid_build_roofline": {"achieved": grid_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": grid_gbs / HBM_PEAK_GBS},
         "scan_roofline": {"n": n_scan, "achieved": scan_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": scan_gbs / HBM_PEAK_GBS, "ms": scan_ms},
-        "roofline": {"kernel": dominant, "bound": "hbm", "ms": dom_ms, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "achieved": None, "frac": None, "traffic": None},
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "ms": dom["ms"], "algorithmic_bytes": dom["bytes"]},
         "cpu_baseline": cpu,
         "copy_calibration": copy,
     }

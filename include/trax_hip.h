@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TRAX_ABI_VERSION 1
+#define TRAX_ABI_VERSION 2
 
 #define TRAX_OK 0
 #define TRAX_ERR_INVALID (-1)  /* bad argument or configuration */
@@ -93,7 +93,14 @@ typedef struct {
   const uint32_t *layer_offset; /* [n_events * n_layers], relative to the event */
 } trax_layout;
 
-/* Scratch of the count/store phases (device pointers, caller-owned). */
+/* The single-call predictor works on tiles of TRAX_PREDICT_TILE pairs; each
+ * tile's candidate slots are staged in a stash slab of
+ * TRAX_PREDICT_TILE_SLOTS uint32 (tiles with more slots walk again). */
+#define TRAX_PREDICT_TILE 64
+#define TRAX_PREDICT_TILE_SLOTS 512
+
+/* Scratch of the count/store phases (device pointers, caller-owned; laid
+ * out by trax_workspace_layout). */
 typedef struct {
   uint32_t *work_offsets;  /* >= max_events * n_triplets + 1 */
   uint32_t *pair_counts;   /* >= max_hits + 1 */
@@ -101,6 +108,10 @@ typedef struct {
   uint32_t *grid_counters; /* >= max_events * n_layers * cells, or NULL */
   uint32_t max_events, max_hits;
   uint64_t pair_capacity;
+  uint64_t cand_capacity;
+  uint32_t *cand_stash;            /* >= ceil(pair_capacity / TRAX_PREDICT_TILE) * TRAX_PREDICT_TILE_SLOTS */
+  unsigned long long *filter_mask; /* >= ceil(cand_capacity / 64) + 8 */
+  uint32_t *filter_counts;         /* >= ceil(cand_capacity / 64) + 9 */
 } trax_workspace;
 
 /* Output buffers of the pair, prediction and filter stages.  Capacities are
@@ -141,16 +152,34 @@ int trax_ctx_get_launch(const trax_ctx *ctx, trax_launch_config *cfg);
 /* Synchronous read of the device status bits; clear != 0 resets them. */
 int trax_ctx_flags(trax_ctx *ctx, void *stream, uint32_t *flags_out, int clear);
 
+/* Kernel timer (no counterpart in the reference, which times whole stages
+ * with host clocks, e.g. TripletThetaPhiPredictor.cpp:44-127): after
+ * trax_ctx_kernel_timer(ctx, id), every eager launch of kernel `id` through
+ * this context is bracketed by HIP events on its launch stream (launches
+ * inside a stream capture are not timed); trax_ctx_kernel_time waits for
+ * them and returns the summed device time and the launch count, then
+ * resets.  id 0 turns the timer off. */
+#define TRAX_KERNEL_GRID_BUILD 1      /* counting sort into the grid */
+#define TRAX_KERNEL_PAIRS 2           /* single-pass pair generation */
+#define TRAX_KERNEL_PREDICT_WALK 3    /* prediction window walk: counts + stash slabs */
+#define TRAX_KERNEL_PREDICT_PLACE 4   /* prediction slabs -> final candidate slots */
+#define TRAX_KERNEL_FILTER_MASK 5     /* filter cuts -> mask words */
+#define TRAX_KERNEL_FILTER_STORE 6    /* filter compaction */
+#define TRAX_KERNEL_SCAN 7            /* every prefix sum the context launches */
+int trax_ctx_kernel_timer(trax_ctx *ctx, int kernel);
+int trax_ctx_kernel_time(trax_ctx *ctx, double *total_ms, uint32_t *launches);
+
 /* minRadiusCurvature(minPt) as TripletConfiguration.cpp:145-152 computes it. */
 float trax_min_radius_curvature(float min_pt);
 
 /* Carves one device allocation at `base` into a workspace for batches of at
- * most max_events events, max_hits hits and pair_capacity pairs, and
- * returns the bytes it needs (call with base = NULL to size it).  The grid
- * counters are only laid out when (n_z+1)(n_phi+1) cells exceed 64 KiB of
- * LDS. */
+ * most max_events events, max_hits hits, pair_capacity pairs and
+ * cand_capacity candidates, and returns the bytes it needs (call with
+ * base = NULL to size it).  The grid counters are only laid out when
+ * (n_z+1)(n_phi+1) cells exceed 64 KiB of LDS. */
 uint64_t trax_workspace_layout(const trax_seed_config *cfg, uint32_t max_events, uint32_t max_hits,
-                               uint64_t pair_capacity, void *base, trax_workspace *ws);
+                               uint64_t pair_capacity, uint64_t cand_capacity, void *base,
+                               trax_workspace *ws);
 
 /* ------------------------------------------------------------- prefix sum */
 /* In-place exclusive uint32 scan (mod 2^32) of n elements; replaces
@@ -204,18 +233,33 @@ int trax_predict_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_ev
                        const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
                        trax_seed_outputs *out, void *stream);
 
-/* Single-call form of the prediction (one pass; reads the pair total on the
- * device): writes out->cands, out->cand_offsets and out->totals[1]. */
+/* Single-call form of the prediction (no host read; the pair total is read
+ * on the device): one walk per pair counts its budget and stages its slots
+ * per tile of pairs in ws->cand_stash, a prefix sum places the budgets, and
+ * a placement pass copies each tile's slots to out->cands; writes
+ * out->cand_offsets and out->totals[1].  Same output as trax_predict_count
+ * + trax_predict_store. */
 int trax_triplet_predict(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
-                         const trax_hits *sorted, const uint32_t *d_grid, trax_seed_outputs *out,
-                         void *stream);
+                         const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                         trax_seed_outputs *out, void *stream);
 
 /* ----------------------------------------------------------------- filter */
-/* TripletThetaPhiFilter::run(...), TripletThetaPhiFilter.h:46-48: cuts and
- * order-preserving compaction in one pass; writes t1/t2/t3, the offsets and
- * out->totals[2]. */
+/* TripletThetaPhiFilter::run(...), TripletThetaPhiFilter.h:46-48, in two
+ * phases.  count: the cuts, one mask bit per candidate, per 64-bit word
+ * popcounts and their prefix sum in the workspace (filterCount,
+ * filterPopCount, PrefixSum); the triplet total lands in out->totals[2].
+ * store: order-preserving compaction into t1/t2/t3 and the per-(event,
+ * layer triplet) offsets (filterStore, filterOffsetStore). */
+int trax_filter_count(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                      const trax_hits *sorted, const trax_workspace *ws, trax_seed_outputs *out,
+                      void *stream);
+int trax_filter_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                      const trax_workspace *ws, trax_seed_outputs *out, void *stream);
+
+/* Both filter phases back to back (no host read). */
 int trax_triplet_filter(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
-                        const trax_hits *sorted, trax_seed_outputs *out, void *stream);
+                        const trax_hits *sorted, const trax_workspace *ws, trax_seed_outputs *out,
+                        void *stream);
 
 /* ------------------------------------------------------------ whole chain */
 /* The four stages back to back with no host synchronisation (the stage

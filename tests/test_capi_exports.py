@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_only_entry_points():
     lib = runtime.lib()
-    assert lib.trax_abi_version() == 1
+    assert lib.trax_abi_version() == 2
     for pt in (0.3, 0.5, 1.0):
         assert np.float32(lib.trax_min_radius_curvature(pt)) == ev.min_radius_curvature(pt)
     # 1 GeV at Bz = 3.8112 T: 87.52 cm (TripletConfiguration.cpp:145-152)
@@ -45,20 +45,31 @@ def test_workspace_layout_sizes():
     sc = ev.config_workload(2, 4).seed_config
     cfg = runtime.seed_config_c(sc)
     ws = runtime.WorkspaceC()
-    n = lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, None, ctypes.byref(ws))
-    need = (4 * 2 + 1) * 4 + 10001 * 4 + 50001 * 4
-    assert need <= n <= need + 4 * 256
+    n = lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, 100000, None, ctypes.byref(ws))
+    words = (100000 + 63) // 64 + 8
+    need = ((4 * 2 + 1) * 4 + 10001 * 4 + 50001 * 4   # work offsets, pair and candidate counts
+            + (50000 + 63) // 64 * 512 * 4            # predictor stash slabs
+            + words * 8 + (words + 1) * 4)            # filter mask words and their counts
+    assert need <= n <= need + 6 * 256
     assert ws.max_events == 4 and ws.max_hits == 10000 and ws.pair_capacity == 50000
+    assert ws.cand_capacity == 100000
     base = 1 << 20
-    lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, ctypes.c_void_p(base),
+    lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, 100000, ctypes.c_void_p(base),
                               ctypes.byref(ws))
-    for p in (ws.work_offsets, ws.pair_counts, ws.cand_counts):
+    ptrs = [ws.work_offsets, ws.pair_counts, ws.cand_counts, ws.cand_stash, ws.filter_mask,
+            ws.filter_counts]
+    for p in ptrs:
         assert p % 256 == 0 and base <= p < base + n
+    # regions in order and disjoint
+    sizes = [(4 * 2 + 1) * 4, 10001 * 4, 50001 * 4, (50000 + 63) // 64 * 512 * 4, words * 8,
+             (words + 1) * 4]
+    for (p, sz), q in zip(zip(ptrs, sizes), ptrs[1:] + [base + n]):
+        assert p + sz <= q
     assert not ws.grid_counters  # 65 x 129 cells fit in LDS
     big = ev.config_workload(2, 4).seed_config
     big.n_z, big.n_phi = 128, 256
     cfgb = runtime.seed_config_c(big)
-    nb = lib.trax_workspace_layout(ctypes.byref(cfgb), 4, 10000, 50000, ctypes.c_void_p(base),
+    nb = lib.trax_workspace_layout(ctypes.byref(cfgb), 4, 10000, 50000, 100000, ctypes.c_void_p(base),
                                    ctypes.byref(ws))
     assert ws.grid_counters and nb >= n + 4 * 4 * 129 * 257 * 4
 

@@ -180,3 +180,50 @@ def test_batched_equals_single_events(ctx):
         mine = g["orig_index"][g["tracklets"][lo:hi]] - base
         alone = s["orig_index"][s["tracklets"]]
         assert np.array_equal(rows_sorted(mine), rows_sorted(alone))
+
+
+def test_single_call_predictor_matches_two_phase(ctx):
+    """The tiled predictor (one walk, scan, placement) writes the same
+    candidate slots as count + scan + store, including pairs whose budget and
+    store list are longer than the lane buffer (they walk again)."""
+    from trax_amd import events as ev
+    from trax_amd import runtime
+    w = ev.config_workload(3, 2)
+    batch = ev.make_batch(w)
+    bufs, db = runtime.run_batch(ctx, w.seed_config, batch)
+    g = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+    written = g["cands"][:, 0] != 0xFFFFFFFF
+    per_pair = np.bincount(g["cands"][written, 0])
+    assert per_pair.max() > runtime.PREDICT_BUFFER  # the re-walk path runs
+    runtime.predict_count(ctx, bufs, db.n_events)
+    runtime.predict_store(ctx, bufs, db.n_events)
+    torch.cuda.synchronize()
+    t = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+    assert np.array_equal(g["cands"], t["cands"])
+    assert np.array_equal(g["cand_offsets"], t["cand_offsets"])
+    assert ctx.flags() == 0
+
+
+@pytest.mark.parametrize("sigma_phi,sigma_z", [(1.5, 0.1), (3.0, 1.0)])
+def test_chain_wide_windows(ctx, sigma_phi, sigma_z):
+    """Wide prediction windows: pairs with more store hits than the lane
+    buffer and tiles whose slots overflow their stash slab both walk again;
+    the chain must still match the oracle, and the single-call predictor the
+    two-phase one."""
+    from trax_amd import events as ev
+    from trax_amd import runtime
+    w = ev.config_workload(1, 1)
+    for t in w.seed_config.triplets:
+        t.sigma_phi, t.sigma_z = sigma_phi, sigma_z
+    batch = ev.make_batch(w)
+    g, o = _compare_chain(ctx, w.seed_config, batch)
+    co = g["cand_offsets"]
+    written = g["cands"][:, 0] != 0xFFFFFFFF
+    assert np.bincount(g["cands"][written, 0]).max() > runtime.PREDICT_BUFFER
+    bufs, db = runtime.run_batch(ctx, w.seed_config, batch)
+    runtime.predict_count(ctx, bufs, db.n_events)
+    runtime.predict_store(ctx, bufs, db.n_events)
+    torch.cuda.synchronize()
+    t = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+    assert np.array_equal(g["cands"], t["cands"])
+    assert np.array_equal(co, t["cand_offsets"])

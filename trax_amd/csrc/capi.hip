@@ -6,12 +6,15 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/trax_hip.h"
 #include "common.h"
 #include "kernels.h"
 
 static_assert(TRAX_MAX_LAYERS == 16 && TRAX_MAX_TRIPLETS == 8, "table sizes");
+static_assert(TRAX_PREDICT_TILE == trax::kPredTile && TRAX_PREDICT_TILE_SLOTS == trax::kTileSlots,
+              "predictor tile shape");
 
 struct trax_ctx {
   int device = 0;
@@ -22,6 +25,13 @@ struct trax_ctx {
   unsigned long long* status = nullptr;
   size_t lds_max = 65536;
   trax_launch_config launch = {4, 256, 256, 8192};
+  // kernel timer: HIP events recorded on the launch stream around every
+  // eager launch of one selected kernel
+  int timed_kernel = 0;
+  std::vector<hipEvent_t> ev_start, ev_stop;
+  size_t ev_used = 0;
+  double timed_ms = 0.0;
+  uint32_t timed_launches = 0;
 };
 
 namespace {
@@ -46,6 +56,37 @@ int hip_fail(trax_ctx* ctx, hipError_t e, const char* where) {
   do {                                                     \
     hipError_t e_ = (expr);                                \
     if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+// Folds the recorded event pairs into the timer's running total.
+int timer_drain(trax_ctx* ctx) {
+  for (size_t i = 0; i < ctx->ev_used; ++i) {
+    TRAX_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_stop[i]));
+    float ms = 0.0f;
+    TRAX_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev_start[i], ctx->ev_stop[i]));
+    ctx->timed_ms += ms;
+    ++ctx->timed_launches;
+  }
+  ctx->ev_used = 0;
+  return TRAX_OK;
+}
+
+// Returns the event pair to bracket this launch with, or -1 when the kernel
+// is not the timed one or the stream is being captured.
+int timer_slot(trax_ctx* ctx, int kernel, hipStream_t s) {
+  if (ctx->timed_kernel != kernel) return -1;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
+  if (ctx->ev_used == ctx->ev_start.size() && timer_drain(ctx) != TRAX_OK) return -1;
+  return (int)ctx->ev_used++;
+}
+
+#define TRAX_TIMED(ctx, kernel, stream, expr)                                       \
+  do {                                                                            \
+    const int slot_ = timer_slot(ctx, kernel, stream);                            \
+    if (slot_ >= 0) TRAX_HIP_CHECK(ctx, hipEventRecord(ctx->ev_start[slot_], stream)); \
+    TRAX_HIP_CHECK(ctx, expr);                                                    \
+    if (slot_ >= 0) TRAX_HIP_CHECK(ctx, hipEventRecord(ctx->ev_stop[slot_], stream));  \
   } while (0)
 
 uint32_t bits_for(uint32_t n) {  // bits needed to hold values 0..n-1
@@ -87,6 +128,9 @@ int make_params(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events, t
   for (uint32_t l = 0; l < TRAX_MAX_LAYERS; ++l) {
     sp->min_radius[l] = cfg->min_radius[l];
     sp->max_radius[l] = cfg->max_radius[l];
+    const float twoR = 2 * sp->min_radius_curvature;
+    sp->acos_min_r[l] = trax_acosf(sp->min_radius[l] / twoR);
+    sp->acos_max_r[l] = trax_acosf(sp->max_radius[l] / twoR);
   }
   for (uint32_t p = 0; p < cfg->n_triplets; ++p) {
     const trax_layer_triplet& t = cfg->triplets[p];
@@ -198,7 +242,41 @@ extern "C" int trax_ctx_destroy(trax_ctx* ctx) {
   if (!ctx) return TRAX_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->block) (void)hipFree(ctx->block);
+  for (hipEvent_t e : ctx->ev_start) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->ev_stop) (void)hipEventDestroy(e);
   delete ctx;
+  return TRAX_OK;
+}
+
+extern "C" int trax_ctx_kernel_timer(trax_ctx* ctx, int kernel) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (kernel < 0 || kernel > TRAX_KERNEL_SCAN) return fail(ctx, TRAX_ERR_INVALID, "unknown kernel id");
+  if (ctx->ev_start.empty()) {
+    const size_t n = 256;
+    for (size_t i = 0; i < n; ++i) {
+      hipEvent_t a, b;
+      TRAX_HIP_CHECK(ctx, hipEventCreate(&a));
+      TRAX_HIP_CHECK(ctx, hipEventCreate(&b));
+      ctx->ev_start.push_back(a);
+      ctx->ev_stop.push_back(b);
+    }
+  }
+  if ((rc = timer_drain(ctx))) return rc;
+  ctx->timed_kernel = kernel;
+  ctx->timed_ms = 0.0;
+  ctx->timed_launches = 0;
+  return TRAX_OK;
+}
+
+extern "C" int trax_ctx_kernel_time(trax_ctx* ctx, double* total_ms, uint32_t* launches) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if ((rc = timer_drain(ctx))) return rc;
+  if (total_ms) *total_ms = ctx->timed_ms;
+  if (launches) *launches = ctx->timed_launches;
+  ctx->timed_ms = 0.0;
+  ctx->timed_launches = 0;
   return TRAX_OK;
 }
 
@@ -250,10 +328,11 @@ extern "C" float trax_min_radius_curvature(float min_pt) {
 }
 
 extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t max_events,
-                                          uint32_t max_hits, uint64_t pair_capacity, void* base,
-                                          trax_workspace* ws) {
+                                          uint32_t max_hits, uint64_t pair_capacity,
+                                          uint64_t cand_capacity, void* base, trax_workspace* ws) {
   if (!cfg) return 0;
   const uint64_t cells = (uint64_t)(cfg->n_z + 1) * (cfg->n_phi + 1);
+  const uint64_t words = trax::filter_mask_words(cand_capacity);
   uint64_t off = 0;
   const uint64_t o_work = off;
   off = align_up(off + ((uint64_t)max_events * cfg->n_triplets + 1) * 4);
@@ -261,6 +340,13 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
   off = align_up(off + ((uint64_t)max_hits + 1) * 4);
   const uint64_t o_cc = off;
   off = align_up(off + (pair_capacity + 1) * 4);
+  const uint64_t o_stash = off;
+  off = align_up(off + (pair_capacity + TRAX_PREDICT_TILE - 1) / TRAX_PREDICT_TILE *
+                           TRAX_PREDICT_TILE_SLOTS * 4);
+  const uint64_t o_mask = off;
+  off = align_up(off + words * 8);
+  const uint64_t o_fc = off;
+  off = align_up(off + (words + 1) * 4);
   uint64_t o_gc = 0;
   const bool need_gc = cells * 4 > 65536;
   if (need_gc) {
@@ -276,6 +362,10 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
     ws->max_events = max_events;
     ws->max_hits = max_hits;
     ws->pair_capacity = pair_capacity;
+    ws->cand_capacity = cand_capacity;
+    ws->cand_stash = b ? reinterpret_cast<uint32_t*>(b + o_stash) : nullptr;
+    ws->filter_mask = b ? reinterpret_cast<unsigned long long*>(b + o_mask) : nullptr;
+    ws->filter_counts = b ? reinterpret_cast<uint32_t*>(b + o_fc) : nullptr;
   }
   return off;
 }
@@ -291,7 +381,7 @@ extern "C" int trax_prefix_sum_dev(trax_ctx* ctx, const uint32_t* d_in, uint32_t
   if ((((uintptr_t)d_in) | ((uintptr_t)d_out)) & 15u)
     return fail(ctx, TRAX_ERR_INVALID, "scan buffers must be 16-byte aligned");
   if (n_max > 0xFFFFFFFFull) return fail(ctx, TRAX_ERR_INVALID, "scan longer than 2^32 elements");
-  TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(d_in, d_out, n_max, d_n, n_add, ctx->ctrl, ctx->status,
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, static_cast<hipStream_t>(stream), trax::launch_scan_u32(d_in, d_out, n_max, d_n, n_add, ctx->ctrl, ctx->status,
                                             static_cast<hipStream_t>(stream), scan_sub(ctx)));
   return TRAX_OK;
 }
@@ -340,7 +430,7 @@ extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const
       return fail(ctx, TRAX_ERR_INVALID, "grid too large for LDS and no workspace grid counters");
     gl.counters = ws->grid_counters;
   }
-  TRAX_HIP_CHECK(ctx, trax::launch_grid_build(sp, gl, static_cast<hipStream_t>(stream)));
+  TRAX_TIMED(ctx, TRAX_KERNEL_GRID_BUILD, static_cast<hipStream_t>(stream), trax::launch_grid_build(sp, gl, static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
 
@@ -361,7 +451,7 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
   TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
                                          ws->max_hits, false, ctx->flags, ctx->launch.pairs_block, s));
-  TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
                                             ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s,
                                             scan_sub(ctx)));
   // pair_offsets[k] = scanned count at the first outer hit of segment k
@@ -400,11 +490,11 @@ extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, u
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals ||
       (!out->pairs && out->pair_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
-  if (((uint64_t)ws->max_hits + 63) / 64 > trax::kStatusWords)  // pair wave tiles
+  if (((uint64_t)ws->max_hits + 255) / 256 > trax::kStatusWords)
     return fail(ctx, TRAX_ERR_INVALID, "too many hits for one launch");
   hipStream_t s = static_cast<hipStream_t>(stream);
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
-  TRAX_HIP_CHECK(ctx, trax::launch_pairs_fused(sp, sorted_view(sorted), d_grid, ws->work_offsets,
+  TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs_fused(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                                reinterpret_cast<uint2*>(out->pairs), out->pair_capacity,
                                                out->pair_offsets, out->totals + 0, ws->max_hits,
                                                ctx->ctrl, ctx->status, ctx->flags, s));
@@ -414,23 +504,36 @@ extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, u
 // ---------------------------------------------------------------- predictor
 extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
                                     const trax_hits* sorted, const uint32_t* d_grid,
-                                    trax_seed_outputs* out, void* stream) {
+                                    const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if (!ws->cand_stash) return fail(ctx, TRAX_ERR_INVALID, "workspace has no predictor stash");
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets ||
       !out->totals || (!out->pairs && out->pair_capacity) || (!out->cands && out->cand_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null prediction-stage buffer");
-  if ((out->pair_capacity + 63) / 64 > trax::kStatusWords)  // prediction wave tiles
-    return fail(ctx, TRAX_ERR_INVALID, "pair capacity too large for one launch");
-  TRAX_HIP_CHECK(ctx, trax::launch_predict_fused(sp, sorted_view(sorted), d_grid,
-                                                 reinterpret_cast<const uint2*>(out->pairs),
-                                                 out->pair_offsets, out->totals + 0, out->pair_capacity,
-                                                 reinterpret_cast<uint2*>(out->cands), out->cand_capacity,
-                                                 out->cand_offsets, out->totals + 1, ctx->ctrl,
-                                                 ctx->status, ctx->flags,
-                                                 static_cast<hipStream_t>(stream)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t K = n_events * cfg->n_triplets;
+  const uint64_t pc = pair_cap(ws, out);
+  const trax::SortedHits h = sorted_view(sorted);
+  const uint2* pairs = reinterpret_cast<const uint2*>(out->pairs);
+  // the stash holds ceil(ws->pair_capacity / 64) slabs; pc <= ws->pair_capacity
+  TRAX_TIMED(ctx, TRAX_KERNEL_PREDICT_WALK, s,
+             trax::launch_predict_walk(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
+                                       ws->cand_counts, ws->cand_stash, s));
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s,
+             trax::launch_scan_u32(ws->cand_counts, ws->cand_counts, pc + 1, out->totals + 0, 1, ctx->ctrl,
+                                   ctx->status, s, scan_sub(ctx)));
+  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->cand_counts, (uint32_t)pc, out->totals + 0,
+                                                  out->pair_offsets, K + 1, out->cand_offsets,
+                                                  out->totals + 1, s));
+  TRAX_TIMED(ctx, TRAX_KERNEL_PREDICT_PLACE, s,
+             trax::launch_predict_place(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
+                                        ws->cand_counts, ws->cand_stash,
+                                        reinterpret_cast<uint2*>(out->cands), out->cand_capacity,
+                                        ctx->flags, s));
   return TRAX_OK;
 }
 
@@ -452,7 +555,7 @@ extern "C" int trax_predict_count(trax_ctx* ctx, const trax_seed_config* cfg, ui
                                            reinterpret_cast<const uint2*>(out->pairs), out->pair_offsets,
                                            out->totals + 0, pc, ws->cand_counts, nullptr, 0, false,
                                            ctx->flags, ctx->launch.predict_block, s));
-  TRAX_HIP_CHECK(ctx, trax::launch_scan_u32(ws->cand_counts, ws->cand_counts, pc + 1, out->totals + 0, 1,
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->cand_counts, ws->cand_counts, pc + 1, out->totals + 0, 1,
                                             ctx->ctrl, ctx->status, s, scan_sub(ctx)));
   TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->cand_counts, (uint32_t)pc, out->totals + 0,
                                                   out->pair_offsets, K + 1, out->cand_offsets,
@@ -481,26 +584,70 @@ extern "C" int trax_predict_store(trax_ctx* ctx, const trax_seed_config* cfg, ui
 }
 
 // ------------------------------------------------------------------- filter
-extern "C" int trax_triplet_filter(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
-                                   const trax_hits* sorted, trax_seed_outputs* out, void* stream) {
+namespace {
+int filter_args(trax_ctx* ctx, const trax_workspace* ws, const trax_seed_outputs* out, bool store) {
+  if (!ws || !ws->filter_mask || !ws->filter_counts)
+    return fail(ctx, TRAX_ERR_INVALID, "workspace has no filter mask");
+  if (!out || !out->cand_offsets || !out->totals || (!out->cands && out->cand_capacity) ||
+      (!out->pairs && out->pair_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null filter-stage buffer");
+  if (store && (!out->tracklet_offsets || ((!out->t1 || !out->t2 || !out->t3) && out->tracklet_capacity)))
+    return fail(ctx, TRAX_ERR_INVALID, "null filter output buffer");
+  if (out->cand_capacity > ws->cand_capacity)
+    return fail(ctx, TRAX_ERR_INVALID, "candidate capacity larger than the workspace's");
+  if (out->cand_capacity >= 0xFFFFFFC0ull)
+    return fail(ctx, TRAX_ERR_INVALID, "candidate capacity must stay below 2^32 - 64");
+  return TRAX_OK;
+}
+}  // namespace
+
+extern "C" int trax_filter_count(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                 const trax_hits* sorted, const trax_workspace* ws,
+                                 trax_seed_outputs* out, void* stream) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if (!sorted || !sorted->x || !sorted->y || !sorted->z || !out || !out->cand_offsets ||
-      !out->tracklet_offsets || !out->totals || (!out->cands && out->cand_capacity) ||
-      (!out->pairs && out->pair_capacity) ||
-      ((!out->t1 || !out->t2 || !out->t3) && out->tracklet_capacity))
+  if ((rc = filter_args(ctx, ws, out, false))) return rc;
+  if (!sorted || !sorted->x || !sorted->y || !sorted->z)
     return fail(ctx, TRAX_ERR_INVALID, "null filter-stage buffer");
-  const uint64_t tiles = (out->cand_capacity + 255) / 256;  // filter wave tiles
-  if (tiles > trax::kStatusWords) return fail(ctx, TRAX_ERR_INVALID, "candidate capacity too large");
-  TRAX_HIP_CHECK(ctx, trax::launch_filter(sp, sorted_view(sorted), reinterpret_cast<const uint2*>(out->pairs),
-                                          reinterpret_cast<const uint2*>(out->cands), out->cand_offsets,
-                                          out->totals + 1, out->cand_capacity, out->t1, out->t2, out->t3,
-                                          out->tracklet_capacity, out->tracklet_offsets, out->totals + 2,
-                                          ctx->ctrl, ctx->status, ctx->flags,
-                                          static_cast<hipStream_t>(stream)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t words = trax::filter_mask_words(ws->cand_capacity);
+  uint32_t* d_nwords = ws->filter_counts + words;
+  TRAX_TIMED(ctx, TRAX_KERNEL_FILTER_MASK, s, trax::launch_filter_mask(sp, sorted_view(sorted), reinterpret_cast<const uint2*>(out->pairs),
+                                               reinterpret_cast<const uint2*>(out->cands), out->cand_offsets,
+                                               out->totals + 1, out->cand_capacity, ws->filter_mask,
+                                               ws->filter_counts, d_nwords, s));
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->filter_counts, ws->filter_counts, words, d_nwords, 0,
+                                            ctx->ctrl, ctx->status, s, scan_sub(ctx)));
+  // total = scanned count at word ceil(n_cands / 64) (the zero word)
+  TRAX_HIP_CHECK(ctx, trax::launch_read_last(ws->filter_counts, d_nwords, out->totals + 2, s));
   return TRAX_OK;
+}
+
+extern "C" int trax_filter_store(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                 const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = filter_args(ctx, ws, out, true))) return rc;
+  TRAX_TIMED(ctx, TRAX_KERNEL_FILTER_STORE, static_cast<hipStream_t>(stream), trax::launch_filter_store(sp, reinterpret_cast<const uint2*>(out->pairs),
+                                                reinterpret_cast<const uint2*>(out->cands), out->cand_offsets,
+                                                out->totals + 1, out->cand_capacity, ws->filter_mask,
+                                                ws->filter_counts, out->t1, out->t2, out->t3,
+                                                out->tracklet_capacity, out->tracklet_offsets,
+                                                out->totals + 2, ctx->flags,
+                                                static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
+extern "C" int trax_triplet_filter(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                   const trax_hits* sorted, const trax_workspace* ws,
+                                   trax_seed_outputs* out, void* stream) {
+  int rc = trax_filter_count(ctx, cfg, n_events, sorted, ws, out, stream);
+  if (rc) return rc;
+  return trax_filter_store(ctx, cfg, n_events, ws, out, stream);
 }
 
 // -------------------------------------------------------------- whole chain
@@ -511,8 +658,8 @@ extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const
   if (rc) return rc;
   const uint32_t E = layout->n_events;
   if ((rc = trax_pairs_beamspot(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
-  if ((rc = trax_triplet_predict(ctx, cfg, E, sorted, d_grid, out, stream))) return rc;
-  return trax_triplet_filter(ctx, cfg, E, sorted, out, stream);
+  if ((rc = trax_triplet_predict(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
+  return trax_triplet_filter(ctx, cfg, E, sorted, ws, out, stream);
 }
 
 // ------------------------------------------------------------- test support

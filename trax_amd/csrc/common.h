@@ -30,6 +30,10 @@ struct SeedParams {
   float dphi_cut[TRAX_MAX_TRIPLETS], sigma_phi[TRAX_MAX_TRIPLETS];
   float tip_cut[TRAX_MAX_TRIPLETS], z0[TRAX_MAX_TRIPLETS];
   float pair_sigma_phi[TRAX_MAX_TRIPLETS];  // pairSpreadPhi bits read as float
+  // acos(radius / (2 minRadiusCurvature)) per layer for the min and max
+  // radius: window terms that depend on the layer only, evaluated once on
+  // the host with the same FP32 definition the kernels use
+  float acos_min_r[TRAX_MAX_LAYERS], acos_max_r[TRAX_MAX_LAYERS];
 };
 
 // Sorted (grid-ordered) hit columns, as left behind by the grid build.
@@ -197,33 +201,10 @@ __device__ __forceinline__ TileScan tile_scan_lookback(uint32_t v, uint32_t tile
   return r;
 }
 
-// Wave-granular variant: each wave is a tile of 64 items with its own ticket
-// and look-back, so waves never wait for one another (irregular per-item work
-// costs the slowest lane of a wave, not of a block).
-__device__ __forceinline__ uint32_t wave_ticket(ScanCtrl* ctrl) {
-  uint32_t t = 0;
-  if (lane_id() == 0) t = __hip_atomic_fetch_add(&ctrl->ticket, 1u, __ATOMIC_RELAXED, TRAX_AGENT);
-  return __builtin_amdgcn_readfirstlane(t);
-}
-
-__device__ __forceinline__ TileScan wave_scan_lookback(uint32_t v, uint32_t tile, uint32_t epoch,
-                                                       unsigned long long* status) {
-  const uint32_t lane = lane_id();
-  const uint32_t incl = wave_inclusive_scan(v);
-  const uint32_t tot = __shfl(incl, 63, 64);
-  uint32_t excl = 0;
-  if (tile == 0) {
-    if (lane == 0) lookback_publish(status, tile, epoch, 2u, tot);
-  } else {
-    if (lane == 0) lookback_publish(status, tile, epoch, 1u, tot);
-    excl = lookback_wave(status, tile, epoch);
-    if (lane == 0) lookback_publish(status, tile, epoch, 2u, excl + tot);
-  }
-  TileScan r;
-  r.base = excl;
-  r.total = tot;
-  r.pos = excl + incl - v;
-  return r;
+// Makes a wave's LDS writes visible to the other lanes of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 // Ticket for the next tile of a persistent launch (block-uniform).

@@ -1,4 +1,4 @@
-// Triplet filter and compaction in one pass.
+// Triplet filter and compaction.
 //
 // Reference: src/algorithms/TripletThetaPhiFilter.h:50-313 and
 // TripletThetaPhiFilter.cpp:10-164: filterCount sets one bit per accepted
@@ -8,15 +8,12 @@
 // filterOffsetStore plus a racy monotonize pass derive per-(event, layer
 // triplet) offsets.
 //
-// Here one kernel does it: each wave takes the next tile of 256 candidates
-// (ticketed, so tiles resolve in order), evaluates the angle cuts, queues the
-// survivors (about 7%) in LDS and runs the circle fit (TIP and radius cuts)
-// on that dense queue, ranks the accepted candidates with wave64 ballots,
-// resolves the tile's output base by decoupled look-back over its
-// predecessors, and writes the (hit1, hit2, hit3) columns in candidate order.
-// All expressions are the reference's, in its order.  The lane that owns the
-// first candidate of an (event, layer triplet) segment writes that segment's
-// output offset.
+// Here: a mask pass (the cuts, one bit per candidate, a popcount per 64-bit
+// word), the exclusive scan of the word counts, and a store pass that
+// writes the accepted candidates' (hit1, hit2, hit3) columns in candidate
+// order and derives the per-segment offsets from the scanned counts and the
+// mask (no monotonize pass needed).  The cuts are the reference's
+// expressions in its order.
 #include "common.h"
 #include "kernels.h"
 
@@ -81,71 +78,62 @@ __device__ __forceinline__ bool filter_circle(const SeedParams& sp, uint32_t lt,
   return (tip <= (tipCut * tipCut)) && (cR >= minRadius);
 }
 
-constexpr uint32_t kFilterSub = 4;
-constexpr uint32_t kFilterTile = 64u * kFilterSub;  // candidates per wave tile
+// Pass 1 (filterCount + filterPopCount, TripletThetaPhiFilter.h:50-166):
+// each wave takes 256 candidates = 4 mask words.  The angle cuts run on all
+// of them; the survivors (about one in fourteen) are queued densely in the
+// wave's LDS slice so that the circle fit runs on full waves.  Each wave
+// writes its 4 mask words (bit = accepted) and their popcounts.  No block
+// barrier and no cross-block ordering: waves run at their own pace.
+constexpr uint32_t kFilterWords = 4;  // mask words per wave task
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
-__global__ __launch_bounds__(256) void k_filter(SeedParams sp, SortedHits h,
-                                                const uint2* __restrict__ pairs,
-                                                const uint2* __restrict__ cands,
-                                                const uint32_t* __restrict__ cand_off,
-                                                const uint32_t* __restrict__ d_ncands,
-                                                uint64_t cand_capacity, uint32_t* __restrict__ t1,
-                                                uint32_t* __restrict__ t2, uint32_t* __restrict__ t3,
-                                                uint64_t trk_capacity,
-                                                uint32_t* __restrict__ trk_off,
-                                                uint32_t* __restrict__ d_ntrk, ScanCtrl* ctrl,
-                                                unsigned long long* status,
-                                                uint32_t* __restrict__ flags) {
+__global__ __launch_bounds__(256) void k_filter_mask(SeedParams sp, SortedHits h,
+                                                     const uint2* __restrict__ pairs,
+                                                     const uint2* __restrict__ cands,
+                                                     const uint32_t* __restrict__ cand_off,
+                                                     const uint32_t* __restrict__ d_ncands,
+                                                     uint64_t cand_capacity,
+                                                     unsigned long long* __restrict__ mask,
+                                                     uint32_t* __restrict__ counts,
+                                                     uint32_t* __restrict__ d_nwords) {
+  __shared__ uint8_t s_queue[4][64 * kFilterWords];
+  __shared__ uint8_t s_pass[4][64 * kFilterWords];
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
-  __shared__ uint16_t s_queue[4][kFilterTile];
-  __shared__ uint8_t s_pass[4][kFilterTile];
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t P = sp.n_triplets, K = sp.n_events * P;
-  const uint32_t epoch = launch_epoch(ctrl);
   const uint32_t* offs = stage_offsets(cand_off, K + 1, s_offs);
   const uint64_t nc64 = *d_ncands;
   const uint32_t nC = (uint32_t)(nc64 < cand_capacity ? nc64 : cand_capacity);
-  uint32_t ntiles = (nC + kFilterTile - 1) / kFilterTile;
-  if (ntiles == 0) ntiles = 1;  // tile 0 always runs so the offsets get written
-  bool overflow = false;
-  while (true) {  // wave tiles of 256 candidates (candidate = tile * 256 + s * 64 + lane)
-    const uint32_t tile = wave_ticket(ctrl);
-    if (tile >= ntiles) break;
-    // stage 1: angle cuts; survivors queued in this wave's LDS slice
-    bool keep[kFilterSub];
-    uint32_t hh1[kFilterSub], hh2[kFilterSub], hh3[kFilterSub];
+  const uint32_t nW = (nC + 63u) / 64u;
+  // words [0, nW] are written (word nW is the zero that makes the exclusive
+  // scan's last element the total)
+  const uint32_t ntasks = (nW + 1u + kFilterWords - 1u) / kFilterWords;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d_nwords = nW + 1u;
+  const uint32_t wstride = gridDim.x * 4u;
+  for (uint32_t task = blockIdx.x * 4u + wave; task < ntasks; task += wstride) {
+    const uint32_t c0 = task * 64u * kFilterWords;
+    bool keep[kFilterWords];
     uint32_t nq = 0;
 #pragma unroll
-    for (uint32_t s = 0; s < kFilterSub; ++s) {
-      const uint32_t c = tile * kFilterTile + s * 64u + lane;
+    for (uint32_t s = 0; s < kFilterWords; ++s) {
+      const uint32_t c = c0 + s * 64u + lane;
       keep[s] = false;
-      hh1[s] = hh2[s] = hh3[s] = 0;
       if (c < nC) {
         const uint2 cd = cands[c];
         if (cd.x != TRAX_UNWRITTEN) {
           const uint2 pr = pairs[cd.x];
-          hh1[s] = pr.x;
-          hh2[s] = pr.y;
-          hh3[s] = cd.y;
           const uint32_t lt = segment_of(offs, K + 1, c) % P;
           keep[s] = filter_angles(sp, lt, h.x[pr.x], h.y[pr.x], h.z[pr.x], h.x[pr.y], h.y[pr.y],
                                   h.z[pr.y], h.x[cd.y], h.y[cd.y], h.z[cd.y]);
         }
       }
       const uint64_t m = __ballot(keep[s]);
-      if (keep[s]) s_queue[wave][nq + lanes_below(m)] = (uint16_t)(s * 64u + lane);
+      if (keep[s]) s_queue[wave][nq + lanes_below(m)] = (uint8_t)(s * 64u + lane);
       nq += (uint32_t)__popcll(m);
     }
     wave_lds_sync();
-    // stage 2: circle fit on the dense queue of survivors
     for (uint32_t j = lane; j < nq; j += 64u) {
       const uint32_t e = s_queue[wave][j];
-      const uint32_t c = tile * kFilterTile + e;
+      const uint32_t c = c0 + e;
       const uint2 cd = cands[c];
       const uint2 pr = pairs[cd.x];
       const uint32_t lt = segment_of(offs, K + 1, c) % P;
@@ -153,56 +141,105 @@ __global__ __launch_bounds__(256) void k_filter(SeedParams sp, SortedHits h,
                                       h.y[cd.y]) ? 1 : 0;
     }
     wave_lds_sync();
-    // stage 3: ordered ranks, wave look-back, store
-    uint32_t local[kFilterSub], running = 0;
 #pragma unroll
-    for (uint32_t s = 0; s < kFilterSub; ++s) {
-      keep[s] = keep[s] && s_pass[wave][s * 64u + lane];
-      const uint64_t m = __ballot(keep[s]);
-      local[s] = running + lanes_below(m);
-      running += (uint32_t)__popcll(m);
-    }
-    const TileScan ts = wave_scan_lookback(lane == 0 ? running : 0u, tile, epoch, status);
-    const uint32_t base = ts.base;
-#pragma unroll
-    for (uint32_t s = 0; s < kFilterSub; ++s) {
-      const uint32_t c = tile * kFilterTile + s * 64u + lane;
-      const uint64_t pos = (uint64_t)base + local[s];
-      if (keep[s]) {
-        if (pos < trk_capacity) {
-          t1[pos] = hh1[s];
-          t2[pos] = hh2[s];
-          t3[pos] = hh3[s];
-        } else {
-          overflow = true;
-        }
+    for (uint32_t s = 0; s < kFilterWords; ++s) {
+      const bool k = keep[s] && s_pass[wave][s * 64u + lane];
+      const uint64_t m = __ballot(k);
+      if (lane == s) {
+        mask[task * kFilterWords + s] = m;
+        counts[task * kFilterWords + s] = (uint32_t)__popcll(m);
       }
-      // segments starting at candidate c: their output offset is this position
-      if (c < nC) write_segment_starts(offs, K, segment_of(offs, K + 1, c), c, (uint32_t)pos, trk_off);
     }
-    if (tile == ntiles - 1 && lane == 0) {
-      const uint32_t total = base + running;
-      for (int64_t k = K; k >= 0 && offs[k] >= nC; --k) trk_off[k] = total;
-      *d_ntrk = total;
-    }
-    wave_lds_sync();  // the queue is reused by the next tile
+    wave_lds_sync();  // s_queue / s_pass are reused by the wave's next task
   }
-  if (overflow) atomicOr(flags, TRAX_FLAG_TRACKLET_OVERFLOW);
-  __syncthreads();
-  launch_done(ctrl, epoch);
 }
 
-hipError_t launch_filter(const SeedParams& sp, const SortedHits& h, const uint2* pairs,
-                         const uint2* cands, const uint32_t* cand_off, const uint32_t* d_ncands,
-                         uint64_t cand_capacity, uint32_t* t1, uint32_t* t2, uint32_t* t3,
-                         uint64_t trk_capacity, uint32_t* trk_off, uint32_t* d_ntrk, ScanCtrl* ctrl,
-                         unsigned long long* status, uint32_t* flags, hipStream_t stream) {
-  uint64_t b = (cand_capacity + kFilterTile - 1) / kFilterTile;
+// Pass 2 (filterStore + filterOffsetStore, TripletThetaPhiFilter.h:168-313):
+// `counts` now holds the exclusive scan of the per-word popcounts.  One lane
+// per mask word writes its accepted candidates' (hit1, hit2, hit3) in
+// candidate order; the per-(event, layer triplet) output offsets are the
+// scanned count at the segment's first candidate's word plus the accepted
+// bits below it in that word.
+__global__ __launch_bounds__(256) void k_filter_store(SeedParams sp, const uint2* __restrict__ pairs,
+                                                      const uint2* __restrict__ cands,
+                                                      const uint32_t* __restrict__ cand_off,
+                                                      const uint32_t* __restrict__ d_ncands,
+                                                      uint64_t cand_capacity,
+                                                      const unsigned long long* __restrict__ mask,
+                                                      const uint32_t* __restrict__ scanned,
+                                                      uint32_t* __restrict__ t1,
+                                                      uint32_t* __restrict__ t2,
+                                                      uint32_t* __restrict__ t3,
+                                                      uint64_t trk_capacity,
+                                                      uint32_t* __restrict__ trk_off,
+                                                      uint32_t* __restrict__ d_ntrk,
+                                                      uint32_t* __restrict__ flags) {
+  const uint32_t K = sp.n_events * sp.n_triplets;
+  const uint64_t nc64 = *d_ncands;
+  const uint32_t nC = (uint32_t)(nc64 < cand_capacity ? nc64 : cand_capacity);
+  const uint32_t nW = (nC + 63u) / 64u;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+  bool overflow = false;
+  for (uint32_t w = first; w < nW; w += stride) {
+    uint64_t m = mask[w];
+    uint64_t pos = scanned[w];
+    while (m) {
+      const uint32_t b = (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      const uint2 cd = cands[w * 64u + b];
+      const uint2 pr = pairs[cd.x];
+      if (pos < trk_capacity) {
+        t1[pos] = pr.x;
+        t2[pos] = pr.y;
+        t3[pos] = cd.y;
+      } else {
+        overflow = true;
+      }
+      ++pos;
+    }
+  }
+  for (uint32_t k = first; k <= K; k += stride) {
+    uint32_t c = cand_off[k];
+    c = c < nC ? c : nC;
+    const uint32_t w = c >> 6, b = c & 63u;
+    const uint64_t below = mask[w] & ((1ull << b) - 1ull);
+    trk_off[k] = scanned[w] + (uint32_t)__popcll(below);
+  }
+  if (first == 0) *d_ntrk = scanned[nW];
+  if (overflow) atomicOr(flags, TRAX_FLAG_TRACKLET_OVERFLOW);
+}
+
+uint64_t filter_mask_words(uint64_t cand_capacity) { return (cand_capacity + 63u) / 64u + 8u; }
+
+hipError_t launch_filter_mask(const SeedParams& sp, const SortedHits& h, const uint2* pairs,
+                              const uint2* cands, const uint32_t* cand_off, const uint32_t* d_ncands,
+                              uint64_t cand_capacity, unsigned long long* mask, uint32_t* counts,
+                              uint32_t* d_nwords, hipStream_t stream) {
+  const uint64_t tasks = ((cand_capacity + 63u) / 64u + kFilterWords) / kFilterWords;
+  uint64_t b = (tasks + 3u) / 4u;
   if (b == 0) b = 1;
-  const uint32_t blocks = (uint32_t)(b < kPersistentBlocks ? b : kPersistentBlocks);
-  hipLaunchKernelGGL(k_filter, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h, pairs, cands, cand_off,
-                     d_ncands, cand_capacity, t1, t2, t3, trk_capacity, trk_off, d_ntrk, ctrl,
-                     status, flags);
+  const uint32_t blocks = (uint32_t)(b < 8192u ? b : 8192u);
+  hipLaunchKernelGGL(k_filter_mask, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
+                     pairs, cands, cand_off, d_ncands, cand_capacity, mask, counts, d_nwords);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_store(const SeedParams& sp, const uint2* pairs, const uint2* cands,
+                               const uint32_t* cand_off, const uint32_t* d_ncands,
+                               uint64_t cand_capacity, const unsigned long long* mask,
+                               const uint32_t* scanned, uint32_t* t1, uint32_t* t2, uint32_t* t3,
+                               uint64_t trk_capacity, uint32_t* trk_off, uint32_t* d_ntrk,
+                               uint32_t* flags, hipStream_t stream) {
+  const uint64_t words = (cand_capacity + 63u) / 64u;
+  const uint64_t K = (uint64_t)sp.n_events * sp.n_triplets + 1u;
+  const uint64_t n = words > K ? words : K;
+  uint64_t b = (n + 255u) / 256u;
+  if (b == 0) b = 1;
+  const uint32_t blocks = (uint32_t)(b < 4096u ? b : 4096u);
+  hipLaunchKernelGGL(k_filter_store, dim3(blocks), dim3(256), 0, stream, sp, pairs, cands, cand_off,
+                     d_ncands, cand_capacity, mask, scanned, t1, t2, t3, trk_capacity, trk_off, d_ntrk,
+                     flags);
   return hipGetLastError();
 }
 

@@ -21,7 +21,7 @@
 #include <stdint.h>
 
 #if defined(__HIPCC__)
-#define TRAX_FP __device__ __forceinline__
+#define TRAX_FP __host__ __device__ __forceinline__
 #else
 #define TRAX_FP static inline
 #endif

@@ -113,7 +113,7 @@ int main(int argc, char** argv) {
     auto pairs = pairGen.run(hits, geom, 256, trip, grid);
     trax::TripletThetaPhiPredictor predictor(ctx, ws, tot);
     auto cands = predictor.run(hits, geom, 256, trip, grid, *pairs);
-    trax::TripletThetaPhiFilter filter(ctx, tot);
+    trax::TripletThetaPhiFilter filter(ctx, ws, tot);
     auto trk = filter.run(hits, grid, *pairs, *cands, 256, trip, geom);
     ctx.finish();
     if (uint32_t f = ctx.flags()) throw trax::Error("device status flags " + std::to_string(f));

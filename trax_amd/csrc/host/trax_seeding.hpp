@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -250,11 +251,18 @@ class TrackletCollection {
 // Shared per-context scratch for the count phases (kept between runs).
 class Workspace {
  public:
-  void ensure(const trax_seed_config& cfg, uint32_t events, uint32_t hits, uint64_t pairs) {
-    if (events <= ws_.max_events && hits <= ws_.max_hits && pairs <= ws_.pair_capacity && mem_.get()) return;
-    const uint64_t n = trax_workspace_layout(&cfg, events, hits, pairs, nullptr, &ws_);
+  void ensure(const trax_seed_config& cfg, uint32_t events, uint32_t hits, uint64_t pairs,
+              uint64_t cands = 1) {
+    if (events <= ws_.max_events && hits <= ws_.max_hits && pairs <= ws_.pair_capacity &&
+        cands <= ws_.cand_capacity && mem_.get())
+      return;
+    events = std::max(events, ws_.max_events);
+    hits = std::max(hits, ws_.max_hits);
+    pairs = std::max(pairs, ws_.pair_capacity);
+    cands = std::max(cands, ws_.cand_capacity);
+    const uint64_t n = trax_workspace_layout(&cfg, events, hits, pairs, cands, nullptr, &ws_);
     mem_.resize(n);
-    trax_workspace_layout(&cfg, events, hits, pairs, mem_.get(), &ws_);
+    trax_workspace_layout(&cfg, events, hits, pairs, cands, mem_.get(), &ws_);
   }
   const trax_workspace* get() const { return &ws_; }
 
@@ -380,15 +388,15 @@ class TripletThetaPhiPredictor : public KernelWrapper {
 
 class TripletThetaPhiFilter : public KernelWrapper {
  public:
-  TripletThetaPhiFilter(Context& c, Totals& t) : KernelWrapper(c), tot(t) {}
-  // TripletThetaPhiFilter.h:46-48; the triplet count bounds the output, so a
-  // first pass sizes it by the candidate count (every candidate can pass).
+  TripletThetaPhiFilter(Context& c, Workspace& w, Totals& t) : KernelWrapper(c), ws(w), tot(t) {}
+  // TripletThetaPhiFilter.h:46-48 / .cpp:10-164: count (cuts, mask
+  // popcounts, prefix sum), read the triplet total, allocate, store.
   std::unique_ptr<TrackletCollection> run(HitCollection& hits, const Grid& grid, const Pairing& pairs,
                                           const Pairing& cands, int /*nThreads*/, const TripletConfigurations& t,
                                           const GeometrySupplement& geom) {
     const trax_seed_config cfg = make_config(grid.config, geom, t);
     const uint32_t E = grid.config.nEvents, P = (uint32_t)t.size();
-    auto trk = std::make_unique<TrackletCollection>(cands.count, E, P);
+    ws.ensure(cfg, E, (uint32_t)hits.size(), 1, cands.count);
     trax_hits s = hits.sorted();
     trax_seed_outputs o{};
     o.pairs = pairs.pairing.get();
@@ -396,16 +404,20 @@ class TripletThetaPhiFilter : public KernelWrapper {
     o.cands = cands.pairing.get();
     o.cand_capacity = cands.count;
     o.cand_offsets = cands.pairingOffsets.get();
-    o.t1 = trk->hit1.get(), o.t2 = trk->hit2.get(), o.t3 = trk->hit3.get();
-    o.tracklet_capacity = cands.count;
-    o.tracklet_offsets = trk->trackletOffsets.get();
     o.totals = tot.d.get();
-    ctx.check(trax_triplet_filter(ctx.ctx(), &cfg, E, &s, &o, ctx.stream()), "TripletThetaPhiFilter::run");
-    trk->count = tot.read(ctx, 2);
+    ctx.check(trax_filter_count(ctx.ctx(), &cfg, E, &s, ws.get(), &o, ctx.stream()), "filter count");
+    const uint32_t n = tot.read(ctx, 2);
+    auto trk = std::make_unique<TrackletCollection>(n, E, P);
+    o.t1 = trk->hit1.get(), o.t2 = trk->hit2.get(), o.t3 = trk->hit3.get();
+    o.tracklet_capacity = n;
+    o.tracklet_offsets = trk->trackletOffsets.get();
+    ctx.check(trax_filter_store(ctx.ctx(), &cfg, E, ws.get(), &o, ctx.stream()), "filter store");
+    trk->count = n;
     return trk;
   }
 
  private:
+  Workspace& ws;
   Totals& tot;
 };
 

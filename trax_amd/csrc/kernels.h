@@ -12,6 +12,10 @@ namespace trax {
 constexpr uint64_t kPersistentBlocks = 2048;
 // Look-back status words reserved per context (tiles of a scan / filter launch).
 constexpr uint64_t kStatusWords = 1ull << 22;
+// Prediction tiles: pairs per tile (one wave) and stash slots per tile
+// (== TRAX_PREDICT_TILE / TRAX_PREDICT_TILE_SLOTS, checked in the C ABI).
+constexpr uint32_t kPredTile = 64;
+constexpr uint32_t kTileSlots = 512;
 
 uint64_t scan_tile_elems();
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
@@ -47,25 +51,38 @@ hipError_t launch_pairs_fused(const SeedParams& sp, const SortedHits& h, const u
                               uint32_t* pair_off, uint32_t* d_total, uint32_t max_work,
                               ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
                               hipStream_t stream);
-hipError_t launch_predict_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                                const uint2* pairs, const uint32_t* pair_off,
-                                const uint32_t* d_npairs, uint64_t pair_capacity, uint2* cands,
-                                uint64_t cand_capacity, uint32_t* cand_off, uint32_t* d_total,
-                                ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                                hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
                                  hipStream_t stream);
+hipError_t launch_read_last(const uint32_t* arr, const uint32_t* d_index, uint32_t* out,
+                            hipStream_t stream);
 hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                           const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                           uint64_t pair_capacity, uint32_t* counts, uint2* cands,
                           uint64_t cand_capacity, bool store, uint32_t* flags, uint32_t block,
                           hipStream_t stream);
-hipError_t launch_filter(const SeedParams& sp, const SortedHits& h, const uint2* pairs,
-                         const uint2* cands, const uint32_t* cand_off, const uint32_t* d_ncands,
-                         uint64_t cand_capacity, uint32_t* t1, uint32_t* t2, uint32_t* t3,
-                         uint64_t trk_capacity, uint32_t* trk_off, uint32_t* d_ntrk, ScanCtrl* ctrl,
-                         unsigned long long* status, uint32_t* flags, hipStream_t stream);
+// Tiled form of the predictor (walk + slab, scan, place).
+hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                               const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
+                               uint64_t pair_capacity, uint32_t* counts, uint32_t* slab,
+                               hipStream_t stream);
+hipError_t launch_predict_place(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                                const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
+                                uint64_t pair_capacity, const uint32_t* scanned, const uint32_t* slab,
+                                uint2* cands, uint64_t cand_capacity, uint32_t* flags,
+                                hipStream_t stream);
+// Filter: mask pass, scan of the word counts, store pass.
+uint64_t filter_mask_words(uint64_t cand_capacity);
+hipError_t launch_filter_mask(const SeedParams& sp, const SortedHits& h, const uint2* pairs,
+                              const uint2* cands, const uint32_t* cand_off, const uint32_t* d_ncands,
+                              uint64_t cand_capacity, unsigned long long* mask, uint32_t* counts,
+                              uint32_t* d_nwords, hipStream_t stream);
+hipError_t launch_filter_store(const SeedParams& sp, const uint2* pairs, const uint2* cands,
+                               const uint32_t* cand_off, const uint32_t* d_ncands,
+                               uint64_t cand_capacity, const unsigned long long* mask,
+                               const uint32_t* scanned, uint32_t* t1, uint32_t* t2, uint32_t* t3,
+                               uint64_t trk_capacity, uint32_t* trk_off, uint32_t* d_ntrk,
+                               uint32_t* flags, hipStream_t stream);
 hipError_t launch_fp32_eval(int op, const float* a, const float* b, float* out, uint64_t n,
                             hipStream_t stream);
 

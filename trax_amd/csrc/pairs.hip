@@ -30,7 +30,8 @@ struct PairWindow {
 
 // PairGeneratorBeamspot.h:110-162 for outer hit (X, Y, Z) with phi = atan2(Y, X).
 __device__ __forceinline__ PairWindow pair_window(const SeedParams& sp, uint32_t p, float X, float Y,
-                                                  float Z, float phi, float minR1, float maxR1) {
+                                                  float Z, float phi, float minR1, float maxR1,
+                                                  float acosMinR1, float acosMaxR1) {
   const float z0 = sp.z0[p], tip = sp.tip_cut[p], sigmaZ = sp.sigma_z[p];
   const float sigmaPhi = sp.pair_sigma_phi[p];
   const float R = sp.min_radius_curvature;
@@ -53,8 +54,10 @@ __device__ __forceinline__ PairWindow pair_window(const SeedParams& sp, uint32_t
   uint32_t zh = trax_f2u(__builtin_floorf((zHigh - sp.min_z) / sp.sector_size_z)) + 1u;
   w.zHighSector = zh < sp.n_z ? zh : sp.n_z;
 
-  tmp = __builtin_fabsf(trax_acosf(r / (2 * R)) - trax_acosf(maxR1 / (2 * R)));
-  float dPhi = __builtin_fabsf(trax_acosf(r / (2 * R)) - trax_acosf(minR1 / (2 * R)));
+  // acos(maxR1 / (2 * R)) and acos(minR1 / (2 * R)) come precomputed per layer
+  const float acosR = trax_acosf(r / (2 * R));
+  tmp = __builtin_fabsf(acosR - acosMaxR1);
+  float dPhi = __builtin_fabsf(acosR - acosMinR1);
   dPhi = trax_b2f(tmp < dPhi) * dPhi + trax_b2f(tmp > dPhi) * tmp;
   tmp = trax_atanf(tip * (r - minR1)) / (r * minR1);
   float phiHigh = trax_atanf(tip * (r - maxR1)) / (r * maxR1);
@@ -163,7 +166,7 @@ __device__ __forceinline__ PairItem pair_item(const SeedParams& sp, const Sorted
   it.g1 = grid + ((uint64_t)e * L + l1) * C;
   it.i = g2[0] + (w - offs[it.k]);
   it.win = pair_window(sp, it.p, h.x[it.i], h.y[it.i], h.z[it.i], h.phi[it.i], sp.min_radius[l1],
-                       sp.max_radius[l1]);
+                       sp.max_radius[l1], sp.acos_min_r[l1], sp.acos_max_r[l1]);
   return it;
 }
 
@@ -202,10 +205,10 @@ __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
   }
 }
 
-// Single-pass form: each wave takes ticketed tiles of 64 outer hits, walks
-// each window once to count, resolves the tile's output base by decoupled
-// look-back, walks again to store (the hits are in cache), and writes the
-// per-segment pair offsets; the last tile writes the total.
+// Single-pass form: each 256-thread block takes ticketed tiles of 256 outer
+// hits, walks each window once to count, resolves the tile's output base by
+// decoupled look-back, walks again to store, and writes the per-segment pair
+// offsets; the last tile writes the total.
 __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h,
                                                      const uint32_t* __restrict__ grid,
                                                      const uint32_t* __restrict__ work_off,
@@ -215,17 +218,17 @@ __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h
                                                      unsigned long long* status,
                                                      uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];
+  __shared__ uint32_t s_tile, s_excl, s_wave[4];
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t epoch = launch_epoch(ctrl);
   const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
   const uint32_t W = work_off[K];
-  uint32_t ntiles = (W + 63u) / 64u;
+  uint32_t ntiles = (W + 255u) / 256u;
   if (ntiles == 0) ntiles = 1;
-  bool overflow = false;
-  while (true) {  // wave tiles of 64 outer hits
-    const uint32_t tile = wave_ticket(ctrl);
+  while (true) {
+    const uint32_t tile = next_tile(ctrl, &s_tile);
     if (tile >= ntiles) break;
-    const uint32_t w = tile * 64u + lane_id();
+    const uint32_t w = tile * 256u + threadIdx.x;
     const bool valid = w < W;
     PairItem it;
     uint32_t n = 0;
@@ -233,9 +236,10 @@ __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h
       it = pair_item(sp, h, grid, offs, w);
       pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
     }
-    const TileScan ts = wave_scan_lookback(n, tile, epoch, status);
+    const TileScan ts = tile_scan_lookback(n, tile, epoch, status, s_wave, &s_excl);
     if (valid) {
       uint64_t pos = ts.pos;
+      bool overflow = false;
       pair_walk(sp, it.g1, it.win, h, [&](uint32_t idx) {
         if (pos < capacity)
           pairs[pos] = make_uint2(idx, it.i);
@@ -244,16 +248,16 @@ __global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h
         ++pos;
         return false;
       });
+      if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
       write_segment_starts(offs, K, it.k, w, ts.pos, pair_off);
     }
-    if (tile == ntiles - 1 && lane_id() == 0) {
+    if (tile == ntiles - 1 && threadIdx.x == 0) {
       const uint32_t total = ts.base + ts.total;
       for (int64_t k = K; k >= 0 && offs[k] >= W; --k) pair_off[k] = total;
       *d_total = total;
     }
+    __syncthreads();  // s_tile / s_wave / s_excl are reused by the next tile
   }
-  if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
-  __syncthreads();
   launch_done(ctrl, epoch);
 }
 
@@ -273,6 +277,19 @@ __global__ __launch_bounds__(256) void k_gather_offsets(const uint32_t* __restri
     out[k] = v;
     if (k == n - 1 && d_total) *d_total = v;
   }
+}
+
+// *out = arr[*d_index - 1]: the last element of a scan whose length is
+// only known on the device.
+__global__ void k_read_last(const uint32_t* __restrict__ arr, const uint32_t* __restrict__ d_index,
+                            uint32_t* __restrict__ out) {
+  if (threadIdx.x == 0) *out = arr[*d_index - 1u];
+}
+
+hipError_t launch_read_last(const uint32_t* arr, const uint32_t* d_index, uint32_t* out,
+                            hipStream_t stream) {
+  hipLaunchKernelGGL(k_read_last, dim3(1), dim3(64), 0, stream, arr, d_index, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t* work_off,

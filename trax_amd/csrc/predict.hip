@@ -30,7 +30,7 @@ template <bool STORE>
 __device__ __forceinline__ PredWindow predict_window(const SeedParams& sp, uint32_t lt, float p1x,
                                                      float p1y, float p1z, float phi1, float p2x,
                                                      float p2y, float p2z, float phi2, float minR,
-                                                     float maxR) {
+                                                     float maxR, float acosMinR, float acosMaxR) {
   const float sigmaZ = sp.sigma_z[lt], sigmaPhi = sp.sigma_phi[lt];
   const float R = sp.min_radius_curvature;
   const float PI = TRAX_PI_F;
@@ -64,8 +64,10 @@ __device__ __forceinline__ PredWindow predict_window(const SeedParams& sp, uint3
   dPhi += (dPhi > PI) ? -2 * PI : (dPhi < -PI) ? 2 * PI : 0;
   dPhi = __builtin_fabsf(dPhi);
   float dHits = __builtin_sqrtf((p2x - p1x) * (p2x - p1x) + (p2y - p1y) * (p2y - p1y));
-  tmp = __builtin_fabsf(trax_acosf(dHits / (2 * R)) - trax_acosf(maxR / (2 * R)));
-  float phiLow = __builtin_fabsf(trax_acosf(dHits / (2 * R)) - trax_acosf(minR / (2 * R)));
+  // acos(maxR / (2 * R)) and acos(minR / (2 * R)) come precomputed per layer
+  const float acosHits = trax_acosf(dHits / (2 * R));
+  tmp = __builtin_fabsf(acosHits - acosMaxR);
+  float phiLow = __builtin_fabsf(acosHits - acosMinR);
   dPhi = trax_maxf(dPhi, trax_maxf(tmp, phiLow));
   float phiHigh = phi + dPhi + sigmaPhi;
   phiLow = phi - dPhi - sigmaPhi;
@@ -117,7 +119,7 @@ __device__ __forceinline__ void predict_walk(const SeedParams& sp, const uint32_
 struct PredItem {
   uint32_t k, lt;
   const uint32_t* g3;
-  float p1x, p1y, p1z, phi1, p2x, p2y, p2z, phi2, minR, maxR;
+  float p1x, p1y, p1z, phi1, p2x, p2y, p2z, phi2, minR, maxR, acosMinR, acosMaxR;
 };
 
 __device__ __forceinline__ PredItem pred_item(const SeedParams& sp, const SortedHits& h,
@@ -136,36 +138,43 @@ __device__ __forceinline__ PredItem pred_item(const SeedParams& sp, const Sorted
   it.p2x = h.x[pr.y], it.p2y = h.y[pr.y], it.p2z = h.z[pr.y], it.phi2 = h.phi[pr.y];
   it.minR = sp.min_radius[layer];
   it.maxR = sp.max_radius[layer];
+  it.acosMinR = sp.acos_min_r[layer];
+  it.acosMaxR = sp.acos_max_r[layer];
   return it;
 }
 
 template <bool STORE>
 __device__ __forceinline__ PredWindow item_window(const SeedParams& sp, const PredItem& it) {
   return predict_window<STORE>(sp, it.lt, it.p1x, it.p1y, it.p1z, it.phi1, it.p2x, it.p2y, it.p2z,
-                               it.phi2, it.minR, it.maxR);
+                               it.phi2, it.minR, it.maxR, it.acosMinR, it.acosMaxR);
 }
 
 // Store walk of pair q into slots [pos, next): the store window's hits until
 // the count's budget is used up (:394), then 0xFFFFFFFF in the unused slots.
+// The first `skip` slots are left alone (already written by the caller).
 __device__ __forceinline__ bool predict_store_walk(const SeedParams& sp, const SortedHits& h,
                                                    const PredItem& it, uint32_t q, uint64_t pos,
                                                    uint64_t next, uint2* __restrict__ cands,
-                                                   uint64_t cand_capacity) {
+                                                   uint64_t cand_capacity, uint32_t skip = 0) {
   bool overflow = false;
-  if (pos < next) {
+  uint64_t seen = 0;  // store-window hits visited
+  if (pos + skip < next) {
     const PredWindow win = item_window<true>(sp, it);
     predict_walk(sp, it.g3, win, h, [&](uint32_t idx) {
-      if (pos < cand_capacity)
-        cands[pos] = make_uint2(q, idx);
-      else
-        overflow = true;
-      ++pos;
-      return pos >= next;
+      const uint64_t d = pos + seen;
+      if (seen >= skip) {
+        if (d < cand_capacity)
+          cands[d] = make_uint2(q, idx);
+        else
+          overflow = true;
+      }
+      ++seen;
+      return pos + seen >= next;
     });
   }
-  for (; pos < next; ++pos) {
-    if (pos < cand_capacity)
-      cands[pos] = make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
+  for (uint64_t d = pos + (seen > skip ? seen : skip); d < next; ++d) {
+    if (d < cand_capacity)
+      cands[d] = make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
     else
       overflow = true;
   }
@@ -202,118 +211,179 @@ __global__ __launch_bounds__(512) void k_predict(SeedParams sp, SortedHits h,
   }
 }
 
-// Single-pass form, ticketed wave tiles of 64 pairs.  One walk over the union of
-// the count and store windows (their phi parts are identical; the z bounds
-// differ by the rOrigin rounding and the zHigh margin) counts the count
-// window's hits and keeps the first kPredBuf store-window hits, in walk
-// order, in LDS.  After the block scan + look-back gives the tile's slot
-// base, each pair writes min(count, store hits) candidates from the buffer
-// (re-walking only if more than kPredBuf are needed) and fills the rest of
-// its budget with 0xFFFFFFFF, exactly as count + scan + store would.
-constexpr uint32_t kPredBuf = 8;
+// Tiled form, used by the batched chain.  A wave owns a tile of 64
+// consecutive pairs.  Pass 1 walks each pair's union of the count and store
+// windows once (their phi parts are identical; the z bounds differ by the
+// rOrigin rounding and the zHigh margin): it counts the count window's hits
+// and buffers the first kPredBuf store-window hits in the lane's own LDS
+// column.  A wave prefix sum of the counts then lays the tile's slots out
+// densely in its stash slab (kTileSlots uint32): the store list, 0xFFFFFFFF
+// for budget slots the store list does not reach, and kSlotRewalk for slots
+// past the buffer.  After the global prefix sum of the counts, pass 2 copies
+// each slab to the tile's final position (slots and pairs of a tile are
+// contiguous in the output) with coalesced reads and writes; only pairs with
+// more than kPredBuf stored hits, or tiles whose slots overflow the slab,
+// walk again.  The result is what count + scan + store give, slot for slot.
+constexpr uint32_t kPredBuf = 16;
+constexpr uint32_t kSlotRewalk = 0xFFFFFFFEu;
 
-__global__ __launch_bounds__(256) void k_predict_fused(SeedParams sp, SortedHits h,
+__device__ __forceinline__ uint32_t predict_count_buffer(const SeedParams& sp, const SortedHits& h,
+                                                         const PredItem& it, uint32_t* s_col,
+                                                         uint32_t* n_store_out) {
+  const uint32_t nPhi = sp.n_phi;
+  const float PI = TRAX_PI_F;
+  const PredWindow wc = item_window<false>(sp, it);
+  const PredWindow ws = item_window<true>(sp, it);
+  const bool wrap = wc.wrap;  // same for both windows
+  const float phiLow = wc.phiLow, phiHigh = wc.phiHigh;
+  const float czLo = wc.zLow, czHi = wc.zHigh, szLo = ws.zLow, szHi = ws.zHigh;
+  const uint32_t zlo = wc.zLowSector < ws.zLowSector ? wc.zLowSector : ws.zLowSector;
+  const uint32_t zhi = wc.zHighSector > ws.zHighSector ? wc.zHighSector : ws.zHighSector;
+  uint32_t n_count = 0, n_store = 0;
+  for (uint32_t zs = zlo; zs < zhi; ++zs) {
+    const bool rc = zs >= wc.zLowSector && zs < wc.zHighSector;
+    const bool rs = zs >= ws.zLowSector && zs < ws.zHighSector;
+    auto one = [&](uint32_t idx, float zi, float actPhi) {
+      const bool ph = (!wrap && (phiLow <= actPhi && actPhi <= phiHigh)) ||
+                      (wrap && ((phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= phiHigh)));
+      n_count += (rc && ph && czLo <= zi && zi <= czHi) ? 1u : 0u;
+      if (rs && ph && szLo <= zi && zi <= szHi) {
+        if (n_store < kPredBuf) s_col[n_store * 256u] = idx;
+        ++n_store;
+      }
+    };
+    auto visit = [&](uint32_t a, uint32_t b) {
+      uint32_t idx = a;
+      for (; idx + 4 <= b; idx += 4) {
+        float zz[4], pp[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          zz[u] = h.z[idx + u];
+          pp[u] = h.phi[idx + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) one(idx + u, zz[u], pp[u]);
+      }
+      for (; idx < b; ++idx) one(idx, h.z[idx], h.phi[idx]);
+    };
+    const uint32_t row = zs * (nPhi + 1);
+    const uint32_t A = it.g3[row + wc.phiLowSector];
+    const uint32_t H = it.g3[row + wc.phiHighSector];
+    if (!wrap) {
+      visit(A, H);
+    } else {
+      visit(A, it.g3[row + nPhi + 1]);
+      visit(it.g3[row], H);
+    }
+  }
+  *n_store_out = n_store;
+  return n_count;
+}
+
+__global__ __launch_bounds__(256) void k_predict_walk(SeedParams sp, SortedHits h,
+                                                      const uint32_t* __restrict__ grid,
+                                                      const uint2* __restrict__ pairs,
+                                                      const uint32_t* __restrict__ pair_off,
+                                                      const uint32_t* __restrict__ d_npairs,
+                                                      uint64_t pair_capacity,
+                                                      uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ slab) {
+  extern __shared__ uint32_t s_offs[];
+  __shared__ uint32_t s_buf[kPredBuf * 256];  // column per thread: s_buf[j * 256 + tid]
+  const uint32_t K = sp.n_events * sp.n_triplets, tid = threadIdx.x, lane = lane_id();
+  const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
+  const uint64_t nP64 = *d_npairs;
+  const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
+  const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
+  const uint32_t nwaves = gridDim.x * (blockDim.x / 64u);
+  uint32_t* s_col = s_buf + tid;
+  for (uint32_t t = blockIdx.x * (blockDim.x / 64u) + (tid >> 6); t < ntiles; t += nwaves) {
+    const uint32_t q = t * kPredTile + lane;
+    uint32_t n_count = 0, n_store = 0;
+    if (q < nP) {
+      const PredItem it = pred_item(sp, h, grid, pairs, offs, q);
+      n_count = predict_count_buffer(sp, h, it, s_col, &n_store);
+      counts[q] = n_count;
+    }
+    const uint32_t incl = wave_inclusive_scan(n_count);
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (total <= kTileSlots) {
+      uint32_t* out = slab + (uint64_t)t * kTileSlots + (incl - n_count);
+      for (uint32_t j = 0; j < n_count; ++j)
+        out[j] = j < n_store ? (j < kPredBuf ? s_col[j * 256u] : kSlotRewalk) : TRAX_UNWRITTEN;
+    }
+  }
+}
+
+// Pass 2: slab -> final slots.  `scanned` is the exclusive scan of the
+// counts over nP + 1 entries.
+__global__ __launch_bounds__(256) void k_predict_place(SeedParams sp, SortedHits h,
                                                        const uint32_t* __restrict__ grid,
                                                        const uint2* __restrict__ pairs,
                                                        const uint32_t* __restrict__ pair_off,
                                                        const uint32_t* __restrict__ d_npairs,
                                                        uint64_t pair_capacity,
+                                                       const uint32_t* __restrict__ scanned,
+                                                       const uint32_t* __restrict__ slab,
                                                        uint2* __restrict__ cands,
                                                        uint64_t cand_capacity,
-                                                       uint32_t* __restrict__ cand_off,
-                                                       uint32_t* __restrict__ d_total, ScanCtrl* ctrl,
-                                                       unsigned long long* status,
                                                        uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];
-  __shared__ uint32_t s_buf[kPredBuf][256];
-  const uint32_t K = sp.n_events * sp.n_triplets, nPhi = sp.n_phi, tid = threadIdx.x;
-  const uint32_t epoch = launch_epoch(ctrl);
+  const uint32_t K = sp.n_events * sp.n_triplets, lane = lane_id();
   const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
-  uint32_t ntiles = (nP + 63u) / 64u;
-  if (ntiles == 0) ntiles = 1;
-  const float PI = TRAX_PI_F;
+  const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
+  const uint32_t nwaves = gridDim.x * (blockDim.x / 64u);
   bool overflow = false;
-  while (true) {  // wave tiles of 64 pairs
-    const uint32_t tile = wave_ticket(ctrl);
-    if (tile >= ntiles) break;
-    const uint32_t q = tile * 64u + lane_id();
-    const bool valid = q < nP;
-    PredItem it;
-    uint32_t n_count = 0, n_store = 0;
-    if (valid) {
-      it = pred_item(sp, h, grid, pairs, offs, q);
-      const PredWindow wc = item_window<false>(sp, it);
-      const PredWindow ws = item_window<true>(sp, it);
-      const bool wrap = wc.wrap;  // same for both windows
-      const float phiLow = wc.phiLow, phiHigh = wc.phiHigh;
-      const uint32_t zlo = wc.zLowSector < ws.zLowSector ? wc.zLowSector : ws.zLowSector;
-      const uint32_t zhi = wc.zHighSector > ws.zHighSector ? wc.zHighSector : ws.zHighSector;
-      for (uint32_t zs = zlo; zs < zhi; ++zs) {
-        const bool rc = zs >= wc.zLowSector && zs < wc.zHighSector;
-        const bool rs = zs >= ws.zLowSector && zs < ws.zHighSector;
-        const float czLo = wc.zLow, czHi = wc.zHigh, szLo = ws.zLow, szHi = ws.zHigh;
-        auto visit = [&](uint32_t a, uint32_t b) {
-          auto one = [&](uint32_t idx, float zi, float actPhi) {
-            const bool ph = (!wrap && (phiLow <= actPhi && actPhi <= phiHigh)) ||
-                            (wrap && ((phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= phiHigh)));
-            n_count += (rc && ph && czLo <= zi && zi <= czHi) ? 1u : 0u;
-            if (rs && ph && szLo <= zi && zi <= szHi) {
-              if (n_store < kPredBuf) s_buf[n_store][tid] = idx;
-              ++n_store;
-            }
-          };
-          uint32_t idx = a;
-          for (; idx + 4 <= b; idx += 4) {
-            float zz[4], pp[4];
+  for (uint32_t t = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); t < ntiles; t += nwaves) {
+    const uint32_t q = t * kPredTile + lane;
+    const uint32_t start = scanned[q < nP ? q : nP];  // first slot of pair q
+    const uint32_t base = __shfl(start, 0, 64);
+    const uint32_t last = t * kPredTile + kPredTile;
+    const uint32_t total = scanned[last < nP ? last : nP] - base;
+    if (total > kTileSlots) {  // slab overflowed: every pair of the tile walks again
+      if (q < nP) {
+        const uint32_t n = scanned[q + 1] - start;
+        if (n) {
+          const PredItem it = pred_item(sp, h, grid, pairs, offs, q);
+          overflow |= predict_store_walk(sp, h, it, q, start, (uint64_t)start + n, cands, cand_capacity);
+        }
+      }
+      continue;
+    }
+    const uint32_t rel = start - base;  // lane l's first slot within the tile
+    for (uint32_t s0 = 0; s0 < total; s0 += 64u) {
+      const uint32_t sl = s0 + lane;
+      // owner lane: the last l with rel(l) <= sl (rel is non-decreasing)
+      uint32_t l = 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              zz[u] = h.z[idx + u];
-              pp[u] = h.phi[idx + u];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) one(idx + u, zz[u], pp[u]);
+      for (uint32_t step = 32; step >= 1; step >>= 1) {
+        const uint32_t c = l + step;
+        const uint32_t rc = __shfl(rel, (int)(c < 64u ? c : 63u), 64);
+        if (c < 64u && rc <= sl) l = c;
+      }
+      const uint32_t j = sl - __shfl(rel, (int)l, 64);
+      if (sl < total) {
+        const uint32_t qo = t * kPredTile + l;
+        const uint32_t v = slab[(uint64_t)t * kTileSlots + sl];
+        const uint64_t d = (uint64_t)base + sl;
+        if (v == kSlotRewalk) {
+          if (j == kPredBuf) {  // this pair's slots past the buffer: walk again
+            const uint32_t n = scanned[qo + 1] - scanned[qo];
+            const PredItem it = pred_item(sp, h, grid, pairs, offs, qo);
+            overflow |= predict_store_walk(sp, h, it, qo, d - j, d - j + n, cands, cand_capacity,
+                                           kPredBuf);
           }
-          for (; idx < b; ++idx) one(idx, h.z[idx], h.phi[idx]);
-        };
-        const uint32_t row = zs * (nPhi + 1);
-        const uint32_t A = it.g3[row + wc.phiLowSector];
-        const uint32_t H = it.g3[row + wc.phiHighSector];
-        if (!wrap) {
-          visit(A, H);
+        } else if (d < cand_capacity) {
+          cands[d] = v != TRAX_UNWRITTEN ? make_uint2(qo, v) : make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
         } else {
-          visit(A, it.g3[row + nPhi + 1]);
-          visit(it.g3[row], H);
+          overflow = true;
         }
       }
-    }
-    const TileScan ts = wave_scan_lookback(n_count, tile, epoch, status);
-    if (valid) {
-      const uint64_t pos = ts.pos;
-      const uint32_t m = n_count < n_store ? n_count : n_store;
-      if (m <= kPredBuf) {
-        for (uint32_t j = 0; j < n_count; ++j) {
-          const uint64_t d = pos + j;
-          if (d < cand_capacity)
-            cands[d] = j < m ? make_uint2(q, s_buf[j][tid]) : make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
-          else
-            overflow = true;
-        }
-      } else {
-        overflow |= predict_store_walk(sp, h, it, q, pos, pos + n_count, cands, cand_capacity);
-      }
-      write_segment_starts(offs, K, it.k, q, ts.pos, cand_off);
-    }
-    if (tile == ntiles - 1 && lane_id() == 0) {
-      const uint32_t total = ts.base + ts.total;
-      for (int64_t k = K; k >= 0 && offs[k] >= nP; --k) cand_off[k] = total;
-      *d_total = total;
     }
   }
   if (overflow) atomicOr(flags, TRAX_FLAG_CAND_OVERFLOW);
-  __syncthreads();
-  launch_done(ctrl, epoch);
 }
 
 hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
@@ -332,18 +402,29 @@ hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint3
   return hipGetLastError();
 }
 
-hipError_t launch_predict_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                                const uint2* pairs, const uint32_t* pair_off,
-                                const uint32_t* d_npairs, uint64_t pair_capacity, uint2* cands,
-                                uint64_t cand_capacity, uint32_t* cand_off, uint32_t* d_total,
-                                ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
+static uint32_t tile_blocks(uint64_t pair_capacity) {
+  const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
+  uint64_t b = (tiles + 3) / 4;  // 4 waves (tiles) per 256-thread block
+  return (uint32_t)(b == 0 ? 1 : (b > 16384u ? 16384u : b));
+}
+
+hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                               const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
+                               uint64_t pair_capacity, uint32_t* counts, uint32_t* slab,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL(k_predict_walk, dim3(tile_blocks(pair_capacity)), dim3(256), offsets_lds_bytes(sp),
+                     stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, slab);
+  return hipGetLastError();
+}
+
+hipError_t launch_predict_place(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                                const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
+                                uint64_t pair_capacity, const uint32_t* scanned, const uint32_t* slab,
+                                uint2* cands, uint64_t cand_capacity, uint32_t* flags,
                                 hipStream_t stream) {
-  uint64_t b = (pair_capacity + 255u) / 256u;
-  if (b == 0) b = 1;
-  const uint32_t blocks = (uint32_t)(b < kPersistentBlocks ? b : kPersistentBlocks);
-  hipLaunchKernelGGL(k_predict_fused, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                     grid, pairs, pair_off, d_npairs, pair_capacity, cands, cand_capacity, cand_off,
-                     d_total, ctrl, status, flags);
+  hipLaunchKernelGGL(k_predict_place, dim3(tile_blocks(pair_capacity)), dim3(256), offsets_lds_bytes(sp),
+                     stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, scanned, slab,
+                     cands, cand_capacity, flags);
   return hipGetLastError();
 }
 

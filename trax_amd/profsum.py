@@ -1,0 +1,136 @@
+"""Summaries of rocprofv3 output for the seeding kernels.
+
+`stats`  : per-kernel table (calls, average / total device time, share, and
+           the launch shape and register / LDS use from the kernel trace) of
+           a `rocprofv3 --kernel-trace --stats --output-format csv` run.
+`traffic`: HBM bytes per launch of each chain kernel from two counter runs
+           (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`), each counter scaled
+           by what it reads for a copy kernel of known size (torch's int32
+           negation of a 256 MiB array: 256 MiB read, 256 MiB written).
+
+Both print text meant to be kept with the command that produced the data:
+
+    python -m trax_amd.profsum stats DIR --command "..."
+    python -m trax_amd.profsum traffic FETCH_DIR WRITE_DIR --workload NAME --command "..."
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+import re
+from collections import defaultdict
+from typing import Dict, List
+
+# Chain kernels by the short names the library's kernel timer uses.
+SHORT = [
+    ("k_grid_build", "grid_build"),
+    ("k_pairs_fused", "pairs"),
+    ("k_pairs<", "pairs_two_phase"),
+    ("k_predict_walk", "predict_walk"),
+    ("k_predict_place", "predict_place"),
+    ("k_predict<", "predict_two_phase"),
+    ("k_filter_mask", "filter_mask"),
+    ("k_filter_store", "filter_store"),
+    ("k_scan_u32", "scan"),
+    ("k_gather_offsets", "gather_offsets"),
+    ("k_pair_plan", "pair_plan"),
+    ("k_read_last", "read_last"),
+]
+COPY_BYTES = 256 << 20  # calibration kernel: bytes read and bytes written
+
+
+def short_name(name: str) -> str:
+    for key, short in SHORT:
+        if key in name:
+            return short
+    if "neg_kernel" in name:
+        return "copy_calibration"
+    return re.sub(r"\(.*", "", name)[:60]
+
+
+def _one(pattern: str) -> str:
+    files = sorted(glob.glob(pattern, recursive=True))
+    if not files:
+        raise SystemExit(f"no file matches {pattern}")
+    return files[0]
+
+
+def _rows(path: str) -> List[dict]:
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def stats(d: str) -> str:
+    st = _rows(_one(os.path.join(d, "**", "*kernel_stats.csv")))
+    shape: Dict[str, dict] = {}
+    tr = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
+    if tr:
+        for r in _rows(tr[0]):
+            n = short_name(r["Kernel_Name"])
+            shape.setdefault(n, {"vgpr": r.get("VGPR_Count"), "agpr": r.get("Accum_VGPR_Count"),
+                                 "sgpr": r.get("SGPR_Count"), "lds": r.get("LDS_Block_Size"),
+                                 "scratch": r.get("Scratch_Size"), "wg": r.get("Workgroup_Size_X"),
+                                 "grid": r.get("Grid_Size_X")})
+    out = ["| kernel | calls | avg us | total ms | % | VGPR | AGPR | SGPR | LDS B | scratch | WG | grid |",
+           "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for r in st:
+        n = short_name(r["Name"])
+        s = shape.get(n, {})
+        out.append(f"| {n} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.2f} | "
+                   f"{s.get('vgpr', '')} | {s.get('agpr', '')} | {s.get('sgpr', '')} | {s.get('lds', '')} | "
+                   f"{s.get('scratch', '')} | {s.get('wg', '')} | {s.get('grid', '')} |")
+    return "\n".join(out)
+
+
+def _counter_means(d: str, counter: str) -> Dict[str, float]:
+    rows = _rows(_one(os.path.join(d, "**", "*counter_collection.csv")))
+    per: Dict[str, List[float]] = defaultdict(list)
+    for r in rows:
+        if r.get("Counter_Name", counter) != counter:
+            continue
+        per[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in per.items()}
+
+
+def traffic(fetch_dir: str, write_dir: str, workload: str) -> str:
+    fetch = _counter_means(fetch_dir, "FETCH_SIZE")
+    write = _counter_means(write_dir, "WRITE_SIZE")
+    if "copy_calibration" not in fetch or "copy_calibration" not in write:
+        raise SystemExit("the calibration copy kernel is missing from the counter runs")
+    kib = COPY_BYTES / 1024.0
+    f_scale = kib / fetch["copy_calibration"]
+    w_scale = kib / write["copy_calibration"]
+    lines = [f"# workload={workload} fetch_scale={f_scale:.4f} write_scale={w_scale:.4f} "
+             f"(copy kernel: FETCH_SIZE {fetch['copy_calibration']:.0f} KiB, WRITE_SIZE "
+             f"{write['copy_calibration']:.0f} KiB for {kib:.0f} KiB each way)"]
+    for k in sorted(set(fetch) & set(write)):
+        fb = fetch[k] * f_scale * 1024.0
+        wb = write[k] * w_scale * 1024.0
+        lines.append(f"kernel={k} fetch_kib={fetch[k]:.1f} write_kib={write[k]:.1f} "
+                     f"hbm_bytes={fb + wb:.0f}")
+    return "\n".join(lines)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("dir")
+    a.add_argument("--command", default="")
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch_dir")
+    b.add_argument("write_dir")
+    b.add_argument("--workload", required=True)
+    b.add_argument("--command", default="")
+    args = ap.parse_args()
+    if args.command:
+        print(f"# command: {args.command}")
+    print(stats(args.dir) if args.what == "stats" else traffic(args.fetch_dir, args.write_dir,
+                                                                 args.workload))
+
+
+if __name__ == "__main__":
+    main()

@@ -18,6 +18,12 @@ from . import events as ev
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libtraxhip.so")
+# Kernel ids of the context's kernel timer (TRAX_KERNEL_* in include/trax_hip.h).
+KERNELS = {"grid_build": 1, "pairs": 2, "predict_walk": 3, "predict_place": 4, "filter_mask": 5,
+           "filter_store": 6, "scan": 7}
+# Store-window hits per pair the single-call predictor buffers before a pair walks again.
+PREDICT_BUFFER = 16
+PREDICT_TILE, PREDICT_TILE_SLOTS = 64, 512  # TRAX_PREDICT_TILE, TRAX_PREDICT_TILE_SLOTS
 
 MAX_LAYERS = 16
 MAX_TRIPLETS = 8
@@ -59,7 +65,8 @@ class LayoutC(C.Structure):
 class WorkspaceC(C.Structure):
     _fields_ = [("work_offsets", _vp), ("pair_counts", _vp), ("cand_counts", _vp),
                 ("grid_counters", _vp), ("max_events", C.c_uint32), ("max_hits", C.c_uint32),
-                ("pair_capacity", C.c_uint64)]
+                ("pair_capacity", C.c_uint64), ("cand_capacity", C.c_uint64),
+                ("cand_stash", _vp), ("filter_mask", _vp), ("filter_counts", _vp)]
 
 
 class LaunchC(C.Structure):
@@ -84,9 +91,11 @@ SIGNATURES = {
     "trax_ctx_flags": (C.c_int, [_vp, _vp, _u32p, C.c_int]),
     "trax_ctx_set_launch": (C.c_int, [_vp, _P(LaunchC)]),
     "trax_ctx_get_launch": (C.c_int, [_vp, _P(LaunchC)]),
+    "trax_ctx_kernel_timer": (C.c_int, [_vp, C.c_int]),
+    "trax_ctx_kernel_time": (C.c_int, [_vp, _P(C.c_double), _u32p]),
     "trax_min_radius_curvature": (C.c_float, [C.c_float]),
-    "trax_workspace_layout": (C.c_uint64, [_P(SeedConfigC), C.c_uint32, C.c_uint32, C.c_uint64, _vp,
-                                           _P(WorkspaceC)]),
+    "trax_workspace_layout": (C.c_uint64, [_P(SeedConfigC), C.c_uint32, C.c_uint32, C.c_uint64,
+                                           C.c_uint64, _vp, _P(WorkspaceC)]),
     "trax_prefix_sum": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),
     "trax_prefix_sum_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint64, _vp, C.c_uint32, _vp]),
     "trax_grid_build": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
@@ -98,13 +107,17 @@ SIGNATURES = {
     "trax_pairs_beamspot": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
                                       _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_triplet_predict": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
-                                       _P(OutputsC), _vp]),
+                                       _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_predict_count": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
                                      _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_predict_store": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
                                      _P(WorkspaceC), _P(OutputsC), _vp]),
-    "trax_triplet_filter": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _P(OutputsC),
-                                      _vp]),
+    "trax_filter_count": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _P(WorkspaceC),
+                                    _P(OutputsC), _vp]),
+    "trax_filter_store": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(WorkspaceC), _P(OutputsC),
+                                    _vp]),
+    "trax_triplet_filter": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _P(WorkspaceC),
+                                      _P(OutputsC), _vp]),
     "trax_seed_chain": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
                                   _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_fp32_eval": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_uint64, _vp]),
@@ -201,6 +214,22 @@ class Context:
         self.check(self.lib.trax_ctx_get_launch(self.h, C.byref(cfg)), "trax_ctx_get_launch")
         return {k: getattr(cfg, k) for k, _ in LaunchC._fields_}
 
+    def time_kernel(self, kernel: str, fn, reps: int):
+        """Average device time (ms) of one launch of `kernel` over `reps`
+        eager calls of fn(), from HIP events the library records around each
+        launch on its stream; returns (ms_per_launch, launches)."""
+        kid = KERNELS[kernel]
+        self.check(self.lib.trax_ctx_kernel_timer(self.h, kid), "trax_ctx_kernel_timer")
+        try:
+            for _ in range(reps):
+                fn()
+            ms, n = C.c_double(0.0), C.c_uint32(0)
+            self.check(self.lib.trax_ctx_kernel_time(self.h, C.byref(ms), C.byref(n)),
+                       "trax_ctx_kernel_time")
+        finally:
+            self.lib.trax_ctx_kernel_timer(self.h, 0)
+        return (ms.value / n.value if n.value else float("nan")), int(n.value)
+
     def flags(self, stream=None, clear: bool = True) -> int:
         v = C.c_uint32(0)
         self.check(self.lib.trax_ctx_flags(self.h, stream_ptr(stream), C.byref(v), int(clear)),
@@ -286,11 +315,11 @@ class ChainBuffers:
         L = lib()
         ws = WorkspaceC()
         nbytes = L.trax_workspace_layout(C.byref(self.cfg), self.max_events, self.max_hits,
-                                         self.pair_capacity, None, C.byref(ws))
+                                         self.pair_capacity, self.cand_capacity, None, C.byref(ws))
         self.ws_mem = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
         self.ws = WorkspaceC()
         L.trax_workspace_layout(C.byref(self.cfg), self.max_events, self.max_hits, self.pair_capacity,
-                                _vp(self.ws_mem.data_ptr()), C.byref(self.ws))
+                                self.cand_capacity, _vp(self.ws_mem.data_ptr()), C.byref(self.ws))
         self.pairs = _empty(2 * self.pair_capacity, "u", device)
         self.pair_offsets = _empty(K1, "u", device)
         self.cands = _empty(2 * self.cand_capacity, "u", device)
@@ -421,7 +450,8 @@ def _stage(ctx, fn, name, bufs, n_events, with_grid=True, stream=None):
         rc = fn(ctx.h, C.byref(cfg), n_events, C.byref(hs), _vp(_ptr(bufs.grid)), C.byref(bufs.ws),
                 C.byref(out), stream_ptr(stream))
     else:
-        rc = fn(ctx.h, C.byref(cfg), n_events, C.byref(hs), C.byref(out), stream_ptr(stream))
+        rc = fn(ctx.h, C.byref(cfg), n_events, C.byref(hs), C.byref(bufs.ws), C.byref(out),
+                stream_ptr(stream))
     ctx.check(rc, name)
 
 
@@ -447,12 +477,20 @@ def pairs_beamspot(ctx, bufs, n_events, stream=None):
 
 
 def triplet_predict(ctx, bufs, n_events, stream=None):
-    """Single-pass prediction stage."""
-    cfg, hs, out = _stage_args(bufs)
-    bufs._keep = (hs, out)
-    ctx.check(ctx.lib.trax_triplet_predict(ctx.h, C.byref(cfg), n_events, C.byref(hs),
-                                           _vp(_ptr(bufs.grid)), C.byref(out), stream_ptr(stream)),
-              "trax_triplet_predict")
+    """Single-call prediction stage (walk + stash, scan, compaction; no host read)."""
+    _stage(ctx, ctx.lib.trax_triplet_predict, "trax_triplet_predict", bufs, n_events, stream=stream)
+
+
+def filter_count(ctx, bufs, n_events, stream=None):
+    _stage(ctx, ctx.lib.trax_filter_count, "trax_filter_count", bufs, n_events, with_grid=False,
+           stream=stream)
+
+
+def filter_store(ctx, bufs, n_events, stream=None):
+    cfg, _, out = _stage_args(bufs)
+    bufs._keep = (out,)
+    ctx.check(ctx.lib.trax_filter_store(ctx.h, C.byref(cfg), n_events, C.byref(bufs.ws), C.byref(out),
+                                        stream_ptr(stream)), "trax_filter_store")
 
 
 def triplet_filter(ctx, bufs, n_events, stream=None):
@@ -478,6 +516,9 @@ def run_plain_event(ctx: Context, bufs: ChainBuffers, db: DeviceBatch) -> int:
     if int(host[1]) > bufs.cand_capacity:
         raise TraxError("candidate capacity exceeded in the plain path")
     predict_store(ctx, bufs, db.n_events)
-    triplet_filter(ctx, bufs, db.n_events)
+    filter_count(ctx, bufs, db.n_events)
     host.copy_(bufs.totals)  # blocking read of the triplet total
+    if int(host[2]) > bufs.tracklet_capacity:
+        raise TraxError("triplet capacity exceeded in the plain path")
+    filter_store(ctx, bufs, db.n_events)
     return int(host[2])
