@@ -76,12 +76,14 @@ typedef struct {
 } trax_seed_config;
 
 /* Hit columns.  Input: x..evt.  Grid-sorted output: all of them, plus
- * orig_index (input index of each sorted hit) and phi (atan2(y, x)). */
+ * orig_index (input index of each sorted hit), phi (atan2(y, x)) and zphi
+ * (2 floats per hit: z, phi interleaved, read by the window walks). */
 typedef struct {
   float *x, *y, *z;
   uint32_t *layer, *det_id, *hit_id, *evt;
   uint32_t *orig_index;
   float *phi;
+  float *zphi;
 } trax_hits;
 
 /* Event layout, device arrays (EventSupplement.h / LayerSupplement.h). */
@@ -136,10 +138,10 @@ typedef struct {
 
 /* Launch shape of the kernels, per context (set by the autotuner). */
 typedef struct {
-  uint32_t grid_waves;    /* waves per grid-build block: 1, 2, 4 or 8 (default 4) */
+  uint32_t grid_waves;    /* waves per grid-build block: 1, 2, 4 or 8 (default 8) */
   uint32_t pairs_block;   /* threads per block, two-phase pair kernels: 64..512 (default 256) */
   uint32_t predict_block; /* threads per block, two-phase prediction kernels (default 256) */
-  uint32_t scan_tile;     /* elements per scan tile: 4096, 8192 or 16384 (default 8192) */
+  uint32_t scan_tile;     /* elements per single-pass scan tile: 4096, 8192 or 16384 (default 4096) */
 } trax_launch_config;
 
 /* ---------------------------------------------------------------- context */
@@ -195,7 +197,8 @@ int trax_prefix_sum_dev(trax_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, ui
 /* ------------------------------------------------------------ grid build */
 /* GridBuilder::run(HitCollection&, nThreads, EventSupplement, LayerSupplement,
  * Grid&), GridBuilder.h:42.  Counting sort of every (event, layer) segment
- * into (z, phi) cells; writes the sorted hit columns (+ orig_index, phi) to
+ * (at most 2^29 - 1 hits per batch) into (z, phi) cells; writes the sorted
+ * hit columns (+ orig_index, phi, zphi) to
  * `sorted` and the cell starts (n_events * n_layers * (n_z+1)(n_phi+1)
  * uint32, global sorted indices) to d_grid.  Within a cell hits keep their
  * input order. */

@@ -56,7 +56,7 @@ def test_launch_shapes_do_not_change_results(ctx):
                 for k in ref:
                     assert np.array_equal(ref[k], key[k]), k
     finally:
-        ctx.set_launch(4, 256, 256, 8192)
+        ctx.set_launch()  # library defaults
 
 
 def test_autotuned_binning_matches_oracle(ctx):
@@ -71,7 +71,7 @@ def test_autotuned_binning_matches_oracle(ctx):
         autotune.apply(ctx, w, res)
         g = _trk(ctx, w.seed_config, batch)
     finally:
-        ctx.set_launch(4, 256, 256, 8192)
+        ctx.set_launch()  # library defaults
     o = bridge.run_chain(w.seed_config, batch)
     assert np.array_equal(g["grid"], o["grid"])
     assert np.array_equal(g["tracklet_offsets"], o["tracklet_offsets"])

@@ -55,7 +55,8 @@ def test_fp32_definitions_bitwise(ctx, op):
 
 # ----------------------------------------------------------------- prefix sum
 @pytest.mark.parametrize("n", [0, 1, 2, 3, 4, 5, 1023, 1024, 1025, 8191, 8192, 8193, 65537,
-                               (1 << 20) - 1, 1 << 20, (1 << 24) + 1])
+                               (1 << 20) - 1, 1 << 20, (1 << 21) - 1, 1 << 21, (1 << 21) + 16385,
+                               (1 << 24) + 1])
 def test_prefix_sum_sizes(ctx, n):
     from trax_amd import runtime
     rng = np.random.default_rng(n)

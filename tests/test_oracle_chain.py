@@ -325,3 +325,21 @@ def test_single_muons_are_all_found():
     assert len(trk) == 10
     assert np.all(hid[:, 0] == hid[:, 1]) and np.all(hid[:, 1] == hid[:, 2])
     assert sorted(hid[:, 0].tolist()) == list(range(1, 11))
+
+
+def test_pu200_results_do_not_depend_on_z_binning():
+    """The high-pile-up workload's pairs, candidates and triplets are the same
+    at 64 and 256 z bins (256 is the autotuned default): the z binning only
+    changes how many hits each window walk visits."""
+    def run(n_z):
+        w = ev.config_workload(3, 1)
+        w.seed_config.n_z = n_z
+        b = ev.make_batch(w)
+        o = bridge.run_chain(w.seed_config, b)
+        oi = o["orig_index"]
+        pairs = {tuple(r) for r in oi[o["pairs"]].tolist()}
+        trk = {tuple(r) for r in oi[o["tracklets"]].tolist()}
+        return len(o["pairs"]), len(o["cands"]), pairs, trk
+    a, b = run(64), run(256)
+    assert a[0] == b[0] and a[1] == b[1]
+    assert a[2] == b[2] and a[3] == b[3] and len(a[3]) > 1000

@@ -26,7 +26,7 @@ WAVES = (1, 2, 4, 8)
 BLOCKS = (64, 128, 256, 512)
 SCAN_TILES = (4096, 8192, 16384)
 PHI_BINS = (64, 128, 256)
-Z_BINS = (32, 64, 128)
+Z_BINS = (64, 128, 256, 512)
 
 
 @dataclass
@@ -59,7 +59,7 @@ def _time(fn, reps, torch) -> float:
 def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps: int = 3):
     """Best launch shape per stage at this binning; returns (launch, stage_ms, chain_ms)."""
     import torch
-    base = dict(grid_waves=4, pairs_block=256, predict_block=256, scan_tile=8192)
+    base = dict(grid_waves=8, pairs_block=256, predict_block=256, scan_tile=4096)
     ctx.set_launch(**base)
     bufs, db = rt.run_batch(ctx, sc, batch)
     E = db.n_events
@@ -123,3 +123,29 @@ def apply(ctx: rt.Context, w: ev.Workload, result: TuneResult) -> ev.Workload:
     ctx.set_launch(**result.launch)
     w.seed_config.n_phi, w.seed_config.n_z = result.n_phi, result.n_z
     return w
+
+
+def main():
+    import argparse
+
+    ap = argparse.ArgumentParser(description="autotune binning and launch shapes for a workload")
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--events", type=int, default=None)
+    ap.add_argument("--phi", type=int, nargs="*", default=list(PHI_BINS))
+    ap.add_argument("--z", type=int, nargs="*", default=list(Z_BINS))
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args()
+    w = ev.config_workload(args.config, args.events)
+    ctx = rt.Context(0)
+    res = tune(ctx, w, phi_bins=args.phi, z_bins=args.z, reps=args.reps,
+               log=lambda m: print(m, flush=True))
+    with open(args.out, "w") as f:
+        f.write(res.to_json())
+    print(f"best: {res.n_phi} phi x {res.n_z} z, chain {res.chain_ms:.3f} ms, launch {res.launch}",
+          flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

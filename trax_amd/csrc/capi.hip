@@ -24,7 +24,7 @@ struct trax_ctx {
   uint32_t* flags = nullptr;
   unsigned long long* status = nullptr;
   size_t lds_max = 65536;
-  trax_launch_config launch = {4, 256, 256, 8192};
+  trax_launch_config launch = {8, 256, 256, 4096};
   // kernel timer: HIP events recorded on the launch stream around every
   // eager launch of one selected kernel
   int timed_kernel = 0;
@@ -164,10 +164,11 @@ trax::SortedHits sorted_view(const trax_hits* h) {
   s.phi = h->phi;
   s.layer = h->layer;
   s.evt = h->evt;
+  s.zphi = reinterpret_cast<const float2*>(h->zphi);
   return s;
 }
 
-bool sorted_ok(const trax_hits* h) { return h && h->x && h->y && h->z && h->phi; }
+bool sorted_ok(const trax_hits* h) { return h && h->x && h->y && h->z && h->phi && h->zphi; }
 
 int check_ctx(trax_ctx* ctx) {
   if (!ctx) return TRAX_ERR_INVALID;
@@ -405,16 +406,18 @@ extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const
   if (!in || !in->x || !in->y || !in->z || !in->layer || !in->det_id || !in->hit_id || !in->evt)
     return fail(ctx, TRAX_ERR_INVALID, "incomplete input hits");
   if (!sorted || !sorted->x || !sorted->y || !sorted->z || !sorted->layer || !sorted->det_id ||
-      !sorted->hit_id || !sorted->evt || !sorted->orig_index || !sorted->phi)
+      !sorted->hit_id || !sorted->evt || !sorted->orig_index || !sorted->phi || !sorted->zphi)
     return fail(ctx, TRAX_ERR_INVALID, "incomplete sorted-hit buffers");
   if (!d_grid || !layout->event_offset || !layout->layer_nhits || !layout->layer_offset)
     return fail(ctx, TRAX_ERR_INVALID, "null grid or layout array");
+  if (layout->n_hits >= (1u << 29)) return fail(ctx, TRAX_ERR_INVALID, "at most 2^29 - 1 hits per batch");
   trax::GridLaunch gl;
   gl.ix = in->x; gl.iy = in->y; gl.iz = in->z;
   gl.ilayer = in->layer; gl.idet = in->det_id; gl.ihid = in->hit_id; gl.ievt = in->evt;
   gl.ox = sorted->x; gl.oy = sorted->y; gl.oz = sorted->z;
   gl.olayer = sorted->layer; gl.odet = sorted->det_id; gl.ohid = sorted->hit_id;
   gl.oevt = sorted->evt; gl.oorig = sorted->orig_index; gl.ophi = sorted->phi;
+  gl.ozphi = reinterpret_cast<float2*>(sorted->zphi);
   gl.event_offset = layout->event_offset;
   gl.layer_nhits = layout->layer_nhits;
   gl.layer_offset = layout->layer_offset;

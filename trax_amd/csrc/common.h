@@ -44,7 +44,14 @@ struct SortedHits {
   const float* __restrict__ phi;  // atan2(y, x), computed once by the grid build
   const uint32_t* __restrict__ layer;
   const uint32_t* __restrict__ evt;
+  const float2* __restrict__ zphi;  // (z, phi) per hit, for the window walks
 };
+
+// (z, phi) of sorted hit idx, addressed with a 32-bit byte offset (the C ABI
+// caps hit counts below 2^29) so the load takes the scalar-base form.
+__device__ __forceinline__ float2 load_zphi(const float2* __restrict__ zphi, uint32_t idx) {
+  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(zphi) + (idx << 3));
+}
 
 // Decoupled look-back control block shared by the scan and the filter
 // compaction.  `ticket` hands out tiles in launch order, `done` counts
@@ -248,56 +255,20 @@ __device__ __forceinline__ void write_segment_starts(const uint32_t* offs, uint3
 // every hit inside the window, in order, and returns true to stop the walk
 // (the prediction store's slot budget).  Returns true if stopped.
 template <typename InWindow, typename F>
-__device__ __forceinline__ bool scan_range(uint32_t a, uint32_t b, const float* __restrict__ z,
-                                           const float* __restrict__ phi, InWindow in_window, F&& f) {
+__device__ __forceinline__ bool scan_range(uint32_t a, uint32_t b, const float2* __restrict__ zphi,
+                                           InWindow in_window, F&& f) {
   uint32_t idx = a;
   for (; idx + 4 <= b; idx += 4) {
-    float zz[4], pp[4];
+    float2 v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      zz[u] = z[idx + u];
-      pp[u] = phi[idx + u];
-    }
+    for (int u = 0; u < 4; ++u) v[u] = load_zphi(zphi, idx + u);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (in_window(zz[u], pp[u]) && f(idx + u)) return true;
+      if (in_window(v[u].x, v[u].y) && f(idx + u)) return true;
   }
-  for (; idx < b; ++idx)
-    if (in_window(z[idx], phi[idx]) && f(idx)) return true;
-  return false;
-}
-
-// Lanes of a wave split into groups of GS consecutive lanes that walk one
-// item's hit ranges together.
-template <int GS>
-struct LaneGroup {
-  uint32_t lane, g, gl;
-  uint64_t mask, below;
-  __device__ __forceinline__ LaneGroup() {
-    lane = lane_id();
-    g = lane / GS;
-    gl = lane % GS;
-    mask = GS == 64 ? ~0ull : (((1ull << GS) - 1ull) << (g * GS));
-    below = (1ull << lane) - 1ull;
-  }
-};
-
-// Group-cooperative walk of hits [a, b) (a, b uniform within the group):
-// f(idx, hit_mask, rank, count) is called once per GS-wide step by every
-// lane of the group; hit_mask marks lanes whose hit is in the window, rank is
-// the number of in-window hits on lower lanes, count the step's total.  It
-// returns true to stop the walk.  Returns true if stopped.
-template <int GS, typename InWindow, typename F>
-__device__ __forceinline__ bool group_scan_range(const LaneGroup<GS>& lg, uint32_t a, uint32_t b,
-                                                 const float* __restrict__ z,
-                                                 const float* __restrict__ phi, InWindow in_window,
-                                                 F&& f) {
-  for (uint32_t base = a; base < b; base += GS) {
-    const uint32_t idx = base + lg.gl;
-    bool p = false;
-    if (idx < b) p = in_window(z[idx], phi[idx]);
-    const uint64_t m = __ballot(p) & lg.mask;
-    if (f(idx, p, (uint32_t)__popcll(m & lg.below), (uint32_t)__popcll(m))) return true;
+  for (; idx < b; ++idx) {
+    const float2 v = load_zphi(zphi, idx);
+    if (in_window(v.x, v.y) && f(idx)) return true;
   }
   return false;
 }

@@ -40,6 +40,7 @@ struct GridArgs {
   uint32_t* __restrict__ oevt;
   uint32_t* __restrict__ oorig;
   float* __restrict__ ophi;
+  float2* __restrict__ ozphi;
   const uint32_t* __restrict__ event_offset;
   const uint32_t* __restrict__ layer_nhits;
   const uint32_t* __restrict__ layer_offset;
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs 
         a.oevt[dst] = a.ievt[base + i];
         a.oorig[dst] = base + i;
         a.ophi[dst] = s_phi[i];
+        a.ozphi[dst] = make_float2(a.iz[base + i], s_phi[i]);
       }
       __syncthreads();
       continue;
@@ -188,6 +190,7 @@ __global__ __launch_bounds__(64 * NW) void k_grid_build(SeedParams sp, GridArgs 
         a.oevt[dst] = a.ievt[base + i];
         a.oorig[dst] = base + i;
         a.ophi[dst] = phi;
+        a.ozphi[dst] = make_float2(z, phi);
       }
     }
     __syncthreads();
@@ -212,7 +215,7 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
   a.ilayer = gl.ilayer; a.idet = gl.idet; a.ihid = gl.ihid; a.ievt = gl.ievt;
   a.ox = gl.ox; a.oy = gl.oy; a.oz = gl.oz;
   a.olayer = gl.olayer; a.odet = gl.odet; a.ohid = gl.ohid; a.oevt = gl.oevt;
-  a.oorig = gl.oorig; a.ophi = gl.ophi;
+  a.oorig = gl.oorig; a.ophi = gl.ophi; a.ozphi = gl.ozphi;
   a.event_offset = gl.event_offset; a.layer_nhits = gl.layer_nhits; a.layer_offset = gl.layer_offset;
   a.n_hits = gl.n_hits;
   a.grid = gl.grid;

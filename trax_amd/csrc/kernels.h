@@ -18,6 +18,8 @@ constexpr uint32_t kPredTile = 64;
 constexpr uint32_t kTileSlots = 512;
 
 uint64_t scan_tile_elems();
+// Scans of at least this many elements use the reduce-then-scan form.
+constexpr uint64_t kReduceScanMin = 1ull << 21;
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
                            uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
                            hipStream_t stream, int sub = 8);
@@ -28,6 +30,7 @@ struct GridLaunch {
   float *ox, *oy, *oz;
   uint32_t *olayer, *odet, *ohid, *oevt, *oorig;
   float* ophi;
+  float2* ozphi;
   const uint32_t *event_offset, *layer_nhits, *layer_offset;
   uint32_t n_hits;
   uint32_t* grid;

@@ -137,11 +137,11 @@ __device__ __forceinline__ void pair_walk(const SeedParams& sp, const uint32_t* 
     const uint32_t H = g1[row + win.phiHighSector];
     const uint32_t E = win.wrap ? zSectorEnd + H - zSectorStart : H;
     // indices j in [A, E) map to j - (j >= zSectorEnd) * len
-    scan_range(A, E < zSectorEnd ? E : zSectorEnd, h.z, h.phi, in_window, f);
+    scan_range(A, E < zSectorEnd ? E : zSectorEnd, h.zphi, in_window, f);
     if (E > zSectorEnd) {
       const uint32_t m = E - len;
-      scan_range(zSectorStart, m < A ? m : A, h.z, h.phi, in_window, f);
-      if (m > zSectorEnd) scan_range(zSectorEnd, m, h.z, h.phi, in_window, f);
+      scan_range(zSectorStart, m < A ? m : A, h.zphi, in_window, f);
+      if (m > zSectorEnd) scan_range(zSectorEnd, m, h.zphi, in_window, f);
     }
   }
 }

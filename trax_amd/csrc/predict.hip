@@ -108,10 +108,10 @@ __device__ __forceinline__ void predict_walk(const SeedParams& sp, const uint32_
     const uint32_t A = g3[row + win.phiLowSector];
     const uint32_t H = g3[row + win.phiHighSector];
     if (!win.wrap) {
-      if (scan_range(A, H, h.z, h.phi, in_window, f)) return;
+      if (scan_range(A, H, h.zphi, in_window, f)) return;
     } else {  // j in [A, zSectorEnd + H - zSectorStart), wrapped at zSectorEnd
-      if (scan_range(A, zSectorEnd, h.z, h.phi, in_window, f)) return;
-      if (scan_range(zSectorStart, H, h.z, h.phi, in_window, f)) return;
+      if (scan_range(A, zSectorEnd, h.zphi, in_window, f)) return;
+      if (scan_range(zSectorStart, H, h.zphi, in_window, f)) return;
     }
   }
 }
@@ -255,16 +255,16 @@ __device__ __forceinline__ uint32_t predict_count_buffer(const SeedParams& sp, c
     auto visit = [&](uint32_t a, uint32_t b) {
       uint32_t idx = a;
       for (; idx + 4 <= b; idx += 4) {
-        float zz[4], pp[4];
+        float2 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          zz[u] = h.z[idx + u];
-          pp[u] = h.phi[idx + u];
-        }
+        for (int u = 0; u < 4; ++u) v[u] = load_zphi(h.zphi, idx + u);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) one(idx + u, zz[u], pp[u]);
+        for (int u = 0; u < 4; ++u) one(idx + u, v[u].x, v[u].y);
       }
-      for (; idx < b; ++idx) one(idx, h.z[idx], h.phi[idx]);
+      for (; idx < b; ++idx) {
+        const float2 v = load_zphi(h.zphi, idx);
+        one(idx, v.x, v.y);
+      }
     };
     const uint32_t row = zs * (nPhi + 1);
     const uint32_t A = it.g3[row + wc.phiLowSector];
@@ -328,9 +328,11 @@ __global__ __launch_bounds__(256) void k_predict_place(SeedParams sp, SortedHits
                                                        uint2* __restrict__ cands,
                                                        uint64_t cand_capacity,
                                                        uint32_t* __restrict__ flags) {
-  extern __shared__ uint32_t s_offs[];
-  const uint32_t K = sp.n_events * sp.n_triplets, lane = lane_id();
-  const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
+  // the pair offsets are only needed by pairs that walk again (rare): read
+  // them from global memory instead of staging them for every block
+  __shared__ uint16_t s_own[4][kTileSlots];
+  const uint32_t lane = lane_id();
+  const uint32_t* offs = pair_off;
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
   const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
@@ -352,36 +354,33 @@ __global__ __launch_bounds__(256) void k_predict_place(SeedParams sp, SortedHits
       }
       continue;
     }
-    const uint32_t rel = start - base;  // lane l's first slot within the tile
-    for (uint32_t s0 = 0; s0 < total; s0 += 64u) {
-      const uint32_t sl = s0 + lane;
-      // owner lane: the last l with rel(l) <= sl (rel is non-decreasing)
-      uint32_t l = 0;
-#pragma unroll
-      for (uint32_t step = 32; step >= 1; step >>= 1) {
-        const uint32_t c = l + step;
-        const uint32_t rc = __shfl(rel, (int)(c < 64u ? c : 63u), 64);
-        if (c < 64u && rc <= sl) l = c;
-      }
-      const uint32_t j = sl - __shfl(rel, (int)l, 64);
-      if (sl < total) {
-        const uint32_t qo = t * kPredTile + l;
-        const uint32_t v = slab[(uint64_t)t * kTileSlots + sl];
-        const uint64_t d = (uint64_t)base + sl;
-        if (v == kSlotRewalk) {
-          if (j == kPredBuf) {  // this pair's slots past the buffer: walk again
-            const uint32_t n = scanned[qo + 1] - scanned[qo];
-            const PredItem it = pred_item(sp, h, grid, pairs, offs, qo);
-            overflow |= predict_store_walk(sp, h, it, qo, d - j, d - j + n, cands, cand_capacity,
-                                           kPredBuf);
-          }
-        } else if (d < cand_capacity) {
-          cands[d] = v != TRAX_UNWRITTEN ? make_uint2(qo, v) : make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
-        } else {
-          overflow = true;
+    // slot -> (owner lane, slot index within the owner's budget), through a
+    // per-wave LDS map: each lane marks its own slots
+    const uint32_t rel = start - base;  // this lane's first slot within the tile
+    const uint32_t n_own = (q < nP ? scanned[q + 1] : start) - start;
+    uint16_t* own = s_own[threadIdx.x >> 6];
+    for (uint32_t j = 0; j < n_own; ++j) own[rel + j] = (uint16_t)((j << 6) | lane);
+    wave_lds_sync();
+    for (uint32_t sl = lane; sl < total; sl += 64u) {
+      const uint32_t o = own[sl];
+      const uint32_t l = o & 63u, j = o >> 6;
+      const uint32_t qo = t * kPredTile + l;
+      const uint32_t v = slab[(uint64_t)t * kTileSlots + sl];
+      const uint64_t d = (uint64_t)base + sl;
+      if (v == kSlotRewalk) {
+        if (j == kPredBuf) {  // this pair's slots past the buffer: walk again
+          const uint32_t n = scanned[qo + 1] - scanned[qo];
+          const PredItem it = pred_item(sp, h, grid, pairs, offs, qo);
+          overflow |= predict_store_walk(sp, h, it, qo, d - j, d - j + n, cands, cand_capacity,
+                                         kPredBuf);
         }
+      } else if (d < cand_capacity) {
+        cands[d] = v != TRAX_UNWRITTEN ? make_uint2(qo, v) : make_uint2(TRAX_UNWRITTEN, TRAX_UNWRITTEN);
+      } else {
+        overflow = true;
       }
     }
+    wave_lds_sync();  // the map is rewritten for the wave's next tile
   }
   if (overflow) atomicOr(flags, TRAX_FLAG_CAND_OVERFLOW);
 }
@@ -422,9 +421,12 @@ hipError_t launch_predict_place(const SeedParams& sp, const SortedHits& h, const
                                 uint64_t pair_capacity, const uint32_t* scanned, const uint32_t* slab,
                                 uint2* cands, uint64_t cand_capacity, uint32_t* flags,
                                 hipStream_t stream) {
-  hipLaunchKernelGGL(k_predict_place, dim3(tile_blocks(pair_capacity)), dim3(256), offsets_lds_bytes(sp),
-                     stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, scanned, slab,
-                     cands, cand_capacity, flags);
+  // one tile per wave: placement is latency-bound, so launch every tile at once
+  const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
+  uint64_t b = (tiles + 3) / 4;
+  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > (1u << 20) ? (1u << 20) : b));
+  hipLaunchKernelGGL(k_predict_place, dim3(blocks), dim3(256), 0, stream, sp, h, grid, pairs, pair_off,
+                     d_npairs, pair_capacity, scanned, slab, cands, cand_capacity, flags);
   return hipGetLastError();
 }
 

@@ -115,6 +115,146 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
   }
 }
 
+// Reduce-then-scan form for large arrays (no atomics, no look-back): pass 1
+// reduces each chunk of kChunk elements to a partial sum, pass 2 scans the
+// partials in one block, pass 3 re-reads each chunk and writes its
+// exclusive scan.  12 B of HBM traffic per element instead of 8, but every
+// block runs independently at streaming bandwidth, and pass 3 visits the
+// chunks in reverse order so that the ones pass 1 read last are still in
+// the Infinity Cache.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kChunkSub = 16;                     // uint4 per thread
+constexpr uint32_t kChunk = 256u * 4u * kChunkSub;    // elements per chunk
+
+__device__ __forceinline__ uint64_t scan_length(uint64_t n_max, const uint32_t* d_n, uint32_t n_add) {
+  if (!d_n) return n_max;
+  const uint64_t dn = (uint64_t)d_n[0] + n_add;
+  return dn < n_max ? dn : n_max;
+}
+
+__device__ __forceinline__ uint4 load4(const uint32_t* in, uint64_t idx, uint64_t n) {
+  uint4 v;
+  if (idx + 4 <= n) {
+    v = *reinterpret_cast<const uint4*>(in + idx);
+  } else {
+    v.x = idx < n ? in[idx] : 0u;
+    v.y = idx + 1 < n ? in[idx + 1] : 0u;
+    v.z = idx + 2 < n ? in[idx + 2] : 0u;
+    v.w = idx + 3 < n ? in[idx + 3] : 0u;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint64_t n_max,
+                                                     const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                     uint32_t* __restrict__ partials) {
+  __shared__ uint32_t s_wave[4];
+  const uint64_t n = scan_length(n_max, d_n, n_add);
+  const uint64_t base = (uint64_t)blockIdx.x * kChunk;
+  if (base >= n) return;
+  const uint32_t tid = threadIdx.x;
+  uint32_t t = 0;
+#pragma unroll
+  for (uint32_t s = 0; s < kChunkSub; ++s) {
+    const uint4 v = load4(in, base + s * 1024u + tid * 4u, n);
+    t += v.x + v.y + v.z + v.w;
+  }
+  t = wave_reduce_sum(t);
+  if (lane_id() == 0) s_wave[tid >> 6] = t;
+  __syncthreads();
+  if (tid == 0) partials[blockIdx.x] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+}
+
+// Exclusive scan of the chunk partials in place (one block of 1024 threads).
+__global__ __launch_bounds__(1024) void k_scan_partials(uint64_t n_max, const uint32_t* __restrict__ d_n,
+                                                        uint32_t n_add, uint32_t* __restrict__ partials) {
+  __shared__ uint32_t s_wave[16];
+  const uint64_t n = scan_length(n_max, d_n, n_add);
+  const uint32_t np = (uint32_t)((n + kChunk - 1) / kChunk);
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < np; b0 += 1024u) {
+    const uint32_t i = b0 + tid;
+    const uint32_t v = i < np ? partials[i] : 0u;
+    const uint32_t incl = wave_inclusive_scan(v);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      const uint32_t c = s_wave[w];
+      wp += (w < (int)wave) ? c : 0u;
+      tot += c;
+    }
+    if (i < np) partials[i] = carry + wp + incl - v;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n_max,
+                                                   const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                   const uint32_t* __restrict__ partials) {
+  __shared__ uint32_t s_wave[kChunkSub][4];
+  const uint64_t n = scan_length(n_max, d_n, n_add);
+  const uint32_t chunk = gridDim.x - 1u - blockIdx.x;
+  const uint64_t base = (uint64_t)chunk * kChunk;
+  if (base >= n) return;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  uint4 v[kChunkSub];
+  uint32_t t[kChunkSub], incl[kChunkSub];
+#pragma unroll
+  for (uint32_t s = 0; s < kChunkSub; ++s) {
+    v[s] = load4(in, base + s * 1024u + tid * 4u, n);
+    t[s] = v[s].x + v[s].y + v[s].z + v[s].w;
+    incl[s] = wave_inclusive_scan(t[s]);
+    if (lane == 63) s_wave[s][wave] = incl[s];
+  }
+  __syncthreads();
+  uint32_t running = partials[chunk];
+#pragma unroll
+  for (uint32_t s = 0; s < kChunkSub; ++s) {
+    uint32_t wp = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t c = s_wave[s][w];
+      wp += (w < (int)wave) ? c : 0u;
+      tot += c;
+    }
+    const uint64_t idx = base + s * 1024u + tid * 4u;
+    uint4 o;
+    o.x = running + wp + incl[s] - t[s];
+    o.y = o.x + v[s].x;
+    o.z = o.y + v[s].y;
+    o.w = o.z + v[s].z;
+    if (idx + 4 <= n) {
+      // streaming store: the result is not read again by this scan
+      u32x4 ov = {o.x, o.y, o.z, o.w};
+      __builtin_nontemporal_store(ov, reinterpret_cast<u32x4*>(out + idx));
+    } else {
+      if (idx < n) out[idx] = o.x;
+      if (idx + 1 < n) out[idx + 1] = o.y;
+      if (idx + 2 < n) out[idx + 2] = o.z;
+      if (idx + 3 < n) out[idx + 3] = o.w;
+    }
+    running += tot;
+  }
+}
+
+// partials: ceil(n_max / kChunk) uint32 of scratch.
+static hipError_t scan_reduce_then_scan(const uint32_t* in, uint32_t* out, uint64_t n_max,
+                                        const uint32_t* d_n, uint32_t n_add, uint32_t* partials,
+                                        hipStream_t stream) {
+  const uint64_t blocks = (n_max + kChunk - 1) / kChunk;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)blocks), dim3(256), 0, stream, in, n_max, d_n, n_add,
+                     partials);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, n_max, d_n, n_add, partials);
+  hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out, n_max, d_n, n_add,
+                     partials);
+  return hipGetLastError();
+}
+
 uint64_t scan_tile_elems() { return 256ull * 4ull * 8ull; }
 
 // Persistent blocks (256 CUs x 8) take tiles in ticket order; measured on
@@ -137,6 +277,11 @@ static hipError_t scan_launch(const uint32_t* in, uint32_t* out, uint64_t n_max,
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
                            uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
                            hipStream_t stream, int sub) {
+  if (n_max >= kReduceScanMin) {
+    // the status words double as the partials of the three-pass form
+    if ((n_max + kChunk - 1) / kChunk > 2 * kStatusWords) return hipErrorInvalidValue;
+    return scan_reduce_then_scan(in, out, n_max, d_n, n_add, reinterpret_cast<uint32_t*>(status), stream);
+  }
   switch (sub) {
     case 4: return scan_launch<4>(in, out, n_max, d_n, n_add, ctrl, status, stream);
     case 16: return scan_launch<16>(in, out, n_max, d_n, n_add, ctrl, status, stream);

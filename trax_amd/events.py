@@ -186,7 +186,11 @@ def config_workload(config_id: int, n_events: Optional[int] = None) -> Workload:
         lo, hi = _band(radii)
         trip = [LayerTriplet(1, 2, 3), LayerTriplet(2, 3, 4), LayerTriplet(3, 4, 5),
                 LayerTriplet(4, 5, 6)]
-        sc = SeedConfig(n_z=64, n_phi=128, min_radius=lo, max_radius=hi, triplets=trip, min_pt=0.5)
+        # 256 z bins: the binning the autotuner (trax_amd.autotune) picks for this
+        # workload; it yields the same pairs, candidates and triplets as 64 z bins
+        # (the barrel spans |z| < 26.6 cm of the grid's fixed +-300 cm, so finer
+        # z bins mainly shorten the window walks)
+        sc = SeedConfig(n_z=256, n_phi=128, min_radius=lo, max_radius=hi, triplets=trip, min_pt=0.5)
         return Workload("batched-pu200-6layer", 3, sc, n_events or 64, radii, pileup=(200, 200),
                         tracks_per_vertex=40, pt_cut=0.5, noise_frac=0.05)
     if config_id == 4:

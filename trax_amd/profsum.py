@@ -3,6 +3,13 @@
 `stats`  : per-kernel table (calls, average / total device time, share, and
            the launch shape and register / LDS use from the kernel trace) of
            a `rocprofv3 --kernel-trace --stats --output-format csv` run.
+`sq`     : instruction-issue view of each kernel from one counter run with
+           SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_WAVE_CYCLES SQ_BUSY_CYCLES
+           SQ_WAIT_INST_ANY SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU
+           GRBM_GUI_ACTIVE: VALU instructions per wave, the share of the
+           SIMDs' VALU issue slots used (one wave64 VALU instruction per SIMD
+           every 4 cycles), the share of lanes active per VALU instruction,
+           and the share of wave time spent waiting on dependencies.
 `traffic`: HBM bytes per launch of each chain kernel from two counter runs
            (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`), each counter scaled
            by what it reads for a copy kernel of known size (torch's int32
@@ -34,6 +41,9 @@ SHORT = [
     ("k_filter_mask", "filter_mask"),
     ("k_filter_store", "filter_store"),
     ("k_scan_u32", "scan"),
+    ("k_scan_reduce", "scan_reduce"),
+    ("k_scan_partials", "scan_partials"),
+    ("k_scan_down", "scan_down"),
     ("k_gather_offsets", "gather_offsets"),
     ("k_pair_plan", "pair_plan"),
     ("k_read_last", "read_last"),
@@ -95,6 +105,26 @@ def _counter_means(d: str, counter: str) -> Dict[str, float]:
     return {k: sum(v) / len(v) for k, v in per.items()}
 
 
+def sq(d: str, xcds: int = 8, simds: int = 1024) -> str:
+    rows = _rows(_one(os.path.join(d, "**", "*counter_collection.csv")))
+    per: Dict[str, Dict[str, List[float]]] = defaultdict(lambda: defaultdict(list))
+    for r in rows:
+        per[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = ["| kernel | waves | VALU insts / wave | VMEM reads / wave | VALU issue % | active lanes % | "
+           "wait % |", "|---|---|---|---|---|---|---|"]
+    for k, cs in per.items():
+        c = {n: sum(v) / len(v) for n, v in cs.items()}
+        if not c.get("SQ_WAVES") or not c.get("GRBM_GUI_ACTIVE") or not c.get("SQ_WAVE_CYCLES"):
+            continue
+        cycles = c["GRBM_GUI_ACTIVE"] / xcds  # GRBM_GUI_ACTIVE sums the XCDs
+        issue = c["SQ_ACTIVE_INST_VALU"] / (simds * cycles / 4.0)
+        lanes = c["SQ_THREAD_CYCLES_VALU"] / (64.0 * c["SQ_ACTIVE_INST_VALU"]) if c["SQ_ACTIVE_INST_VALU"] else 0
+        out.append(f"| {k} | {c['SQ_WAVES']:.0f} | {c['SQ_INSTS_VALU'] / c['SQ_WAVES']:.0f} | "
+                   f"{c['SQ_INSTS_VMEM_RD'] / c['SQ_WAVES']:.0f} | {100 * issue:.0f} | {100 * lanes:.0f} | "
+                   f"{100 * c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:.0f} |")
+    return "\n".join(out)
+
+
 def traffic(fetch_dir: str, write_dir: str, workload: str) -> str:
     fetch = _counter_means(fetch_dir, "FETCH_SIZE")
     write = _counter_means(write_dir, "WRITE_SIZE")
@@ -120,6 +150,9 @@ def main():
     a = sub.add_parser("stats")
     a.add_argument("dir")
     a.add_argument("--command", default="")
+    c = sub.add_parser("sq")
+    c.add_argument("dir")
+    c.add_argument("--command", default="")
     b = sub.add_parser("traffic")
     b.add_argument("fetch_dir")
     b.add_argument("write_dir")
@@ -128,8 +161,12 @@ def main():
     args = ap.parse_args()
     if args.command:
         print(f"# command: {args.command}")
-    print(stats(args.dir) if args.what == "stats" else traffic(args.fetch_dir, args.write_dir,
-                                                                 args.workload))
+    if args.what == "stats":
+        print(stats(args.dir))
+    elif args.what == "sq":
+        print(sq(args.dir))
+    else:
+        print(traffic(args.fetch_dir, args.write_dir, args.workload))
 
 
 if __name__ == "__main__":

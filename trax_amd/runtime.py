@@ -54,7 +54,7 @@ class SeedConfigC(C.Structure):
 
 class HitsC(C.Structure):
     _fields_ = [("x", _vp), ("y", _vp), ("z", _vp), ("layer", _vp), ("det_id", _vp),
-                ("hit_id", _vp), ("evt", _vp), ("orig_index", _vp), ("phi", _vp)]
+                ("hit_id", _vp), ("evt", _vp), ("orig_index", _vp), ("phi", _vp), ("zphi", _vp)]
 
 
 class LayoutC(C.Structure):
@@ -205,7 +205,7 @@ class Context:
             msg = self.lib.trax_ctx_last_error(self.h)
             raise TraxError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def set_launch(self, grid_waves=4, pairs_block=256, predict_block=256, scan_tile=8192):
+    def set_launch(self, grid_waves=8, pairs_block=256, predict_block=256, scan_tile=4096):
         cfg = LaunchC(grid_waves, pairs_block, predict_block, scan_tile)
         self.check(self.lib.trax_ctx_set_launch(self.h, C.byref(cfg)), "trax_ctx_set_launch")
 
@@ -310,6 +310,7 @@ class ChainBuffers:
         n = self.max_hits
         self.sorted = {k: _empty(n, "f" if k in ("x", "y", "z", "phi") else "u", device)
                        for k in ("x", "y", "z", "layer", "det_id", "hit_id", "evt", "orig_index", "phi")}
+        self.sorted["zphi"] = _empty(2 * n, "f", device)
         K1 = self.max_events * len(sc.triplets) + 1
         self.grid = _empty(self.max_events * sc.n_layers * sc.cells_per_layer, "u", device)
         L = lib()
@@ -331,7 +332,7 @@ class ChainBuffers:
     def sorted_c(self) -> HitsC:
         s = self.sorted
         return HitsC(*[_ptr(s[k]) for k in ("x", "y", "z", "layer", "det_id", "hit_id", "evt",
-                                            "orig_index", "phi")])
+                                            "orig_index", "phi", "zphi")])
 
     def outputs_c(self) -> OutputsC:
         return OutputsC(_ptr(self.pairs), self.pair_capacity, _ptr(self.pair_offsets),
