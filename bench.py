@@ -235,6 +235,25 @@ def main():
                 "gbs": 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9}
         log(f"copy 256 MiB: {copy_ms:.3f} ms, {copy['gbs']:.0f} GB/s")
 
+    # ---- PCIe: what a caller starting from host buffers adds (pinned copies of
+    # the input columns and layout up, the triplet columns and offsets down)
+    host_in = {k: t.cpu().pin_memory() for k, t in db.cols.items()}
+    h2d_ms = time_events(lambda: [db.cols[k].copy_(host_in[k], non_blocking=True) for k in host_in],
+                         5, torch)
+    host_out = [torch.empty(n_trk, dtype=torch.int32, pin_memory=True) for _ in range(3)]
+    host_off = torch.empty(bufs.tracklet_offsets.numel(), dtype=torch.int32, pin_memory=True)
+
+    def d2h():
+        for i in range(3):
+            host_out[i].copy_(bufs.t[i][:n_trk], non_blocking=True)
+        host_off.copy_(bufs.tracklet_offsets, non_blocking=True)
+    d2h_ms = time_events(d2h, 5, torch)
+    in_bytes = sum(t.numel() * t.element_size() for t in host_in.values())
+    pcie = {"h2d_ms": h2d_ms, "d2h_ms": d2h_ms, "h2d_bytes": in_bytes,
+            "d2h_bytes": 12 * n_trk + host_off.numel() * 4,
+            "events_per_sec": batch.n_events / ((ms + h2d_ms + d2h_ms) * 1e-3)}
+    log(f"pcie: up {h2d_ms:.3f} ms, down {d2h_ms:.3f} ms, inclusive {pcie['events_per_sec']:.0f} events/s")
+
     # ---- plain path: one event per launch sequence, host reads between stages
     plain = None
     if not args.no_plain:
@@ -290,6 +309,7 @@ def main():
         "counts": {"pairs": n_pairs, "candidates": n_cands, "triplets": n_trk},
         "plain": plain,
         "batched_over_plain": (ev_s / plain["events_per_sec"]) if plain else None,
+        "pcie_inclusive": pcie,
         "stages_ms": st,
         "kernels": kern,
         "grid_build_roofline": {"achieved": grid_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

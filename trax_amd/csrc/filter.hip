@@ -19,27 +19,31 @@
 
 namespace trax {
 
-// filterCount predicate, TripletThetaPhiFilter.h:68-147, split in two: the
-// theta-ratio and phi cuts (cheap, most candidates stop here) and the
+// filterCount predicate, TripletThetaPhiFilter.h:68-147, split in three:
+// the phi-difference cut (two atan2; about one candidate in seven passes),
+// the theta-ratio cut (two atan2, two square roots, a division) and the
 // circle-fit TIP and radius cuts.  The reference evaluates all of them and
-// ANDs the results; the cuts are pure functions, so the product is the same.
-__device__ __forceinline__ bool filter_angles(const SeedParams& sp, uint32_t lt, float x1, float y1,
-                                              float z1, float x2, float y2, float z2, float x3,
-                                              float y3, float z3) {
-  const float dThetaCut = sp.dtheta_cut[lt], dPhiCut = sp.dphi_cut[lt];
+// ANDs the results; the cuts are pure functions, so evaluating them in any
+// order, each only on the previous one's survivors, gives the same product.
+__device__ __forceinline__ bool filter_phi(const SeedParams& sp, uint32_t lt, float x1, float y1,
+                                           float x2, float y2, float x3, float y3) {
+  const float dPhiCut = sp.dphi_cut[lt];
   const float PI = TRAX_PI_F;
-  bool valid = true;
-  float angle1 = trax_atan2f(__builtin_sqrtf((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1)), (z2 - z1));
-  float angle2 = trax_atan2f(__builtin_sqrtf((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2)), (z3 - z2));
-  float delta = __builtin_fabsf(angle2 / angle1);
-  valid = valid && (1 - dThetaCut <= delta && delta <= 1 + dThetaCut);
-
-  angle1 = trax_atan2f((y2 - y1), (x2 - x1));
-  angle2 = trax_atan2f((y3 - y2), (x3 - x2));
-  delta = angle2 - angle1;
+  const float angle1 = trax_atan2f((y2 - y1), (x2 - x1));
+  const float angle2 = trax_atan2f((y3 - y2), (x3 - x2));
+  float delta = angle2 - angle1;
   delta += (delta > PI) ? -2 * PI : (delta < -PI) ? 2 * PI : 0;
-  valid = valid && (__builtin_fabsf(delta) <= dPhiCut);
-  return valid;
+  return __builtin_fabsf(delta) <= dPhiCut;
+}
+
+__device__ __forceinline__ bool filter_theta(const SeedParams& sp, uint32_t lt, float x1, float y1,
+                                             float z1, float x2, float y2, float z2, float x3,
+                                             float y3, float z3) {
+  const float dThetaCut = sp.dtheta_cut[lt];
+  const float angle1 = trax_atan2f(__builtin_sqrtf((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1)), (z2 - z1));
+  const float angle2 = trax_atan2f(__builtin_sqrtf((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2)), (z3 - z2));
+  const float delta = __builtin_fabsf(angle2 / angle1);
+  return 1 - dThetaCut <= delta && delta <= 1 + dThetaCut;
 }
 
 // Circle through the three hits from the paraboloid (x, y, x^2 + y^2)
@@ -79,12 +83,36 @@ __device__ __forceinline__ bool filter_circle(const SeedParams& sp, uint32_t lt,
 }
 
 // Pass 1 (filterCount + filterPopCount, TripletThetaPhiFilter.h:50-166):
-// each wave takes 256 candidates = 4 mask words.  The angle cuts run on all
-// of them; the survivors (about one in fourteen) are queued densely in the
-// wave's LDS slice so that the circle fit runs on full waves.  Each wave
-// writes its 4 mask words (bit = accepted) and their popcounts.  No block
-// barrier and no cross-block ordering: waves run at their own pace.
+// each wave takes 256 candidates = 4 mask words.  The phi cut runs on all of
+// them; its survivors are queued densely in the wave's LDS slice for the
+// theta cut, whose survivors are queued again for the circle fit, so the
+// costlier stages run on full waves.  Each wave writes its 4 mask words
+// (bit = accepted) and their popcounts.  No block barrier and no cross-block
+// ordering: waves run at their own pace.
 constexpr uint32_t kFilterWords = 4;  // mask words per wave task
+
+// One filter stage over a wave's dense LDS queue of candidate numbers
+// (0..255 within the task): keeps those that pass `cut`, compacted in order
+// into `out`; returns how many.
+template <typename Cut>
+__device__ __forceinline__ uint32_t filter_stage(const uint8_t* in, uint32_t n_in, uint8_t* out,
+                                                 Cut&& cut) {
+  uint32_t n_out = 0;
+  for (uint32_t j0 = 0; j0 < n_in; j0 += 64u) {
+    const uint32_t j = j0 + lane_id();
+    uint8_t e = 0;
+    bool ok = false;
+    if (j < n_in) {
+      e = in[j];
+      ok = cut(e);
+    }
+    const uint64_t m = __ballot(ok);
+    if (ok) out[n_out + lanes_below(m)] = e;
+    n_out += (uint32_t)__popcll(m);
+  }
+  wave_lds_sync();
+  return n_out;
+}
 
 __global__ __launch_bounds__(256) void k_filter_mask(SeedParams sp, SortedHits h,
                                                      const uint2* __restrict__ pairs,
@@ -95,7 +123,8 @@ __global__ __launch_bounds__(256) void k_filter_mask(SeedParams sp, SortedHits h
                                                      unsigned long long* __restrict__ mask,
                                                      uint32_t* __restrict__ counts,
                                                      uint32_t* __restrict__ d_nwords) {
-  __shared__ uint8_t s_queue[4][64 * kFilterWords];
+  __shared__ uint8_t s_q1[4][64 * kFilterWords];
+  __shared__ uint8_t s_q2[4][64 * kFilterWords];
   __shared__ uint8_t s_pass[4][64 * kFilterWords];
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
@@ -109,48 +138,60 @@ __global__ __launch_bounds__(256) void k_filter_mask(SeedParams sp, SortedHits h
   const uint32_t ntasks = (nW + 1u + kFilterWords - 1u) / kFilterWords;
   if (blockIdx.x == 0 && threadIdx.x == 0) *d_nwords = nW + 1u;
   const uint32_t wstride = gridDim.x * 4u;
+  uint8_t* q1 = s_q1[wave];
+  uint8_t* q2 = s_q2[wave];
+  uint8_t* pass = s_pass[wave];
   for (uint32_t task = blockIdx.x * 4u + wave; task < ntasks; task += wstride) {
     const uint32_t c0 = task * 64u * kFilterWords;
-    bool keep[kFilterWords];
+    // stage 1: phi cut on every candidate of the task
     uint32_t nq = 0;
 #pragma unroll
     for (uint32_t s = 0; s < kFilterWords; ++s) {
       const uint32_t c = c0 + s * 64u + lane;
-      keep[s] = false;
+      pass[s * 64u + lane] = 0;
+      bool keep = false;
       if (c < nC) {
         const uint2 cd = cands[c];
         if (cd.x != TRAX_UNWRITTEN) {
           const uint2 pr = pairs[cd.x];
           const uint32_t lt = segment_of(offs, K + 1, c) % P;
-          keep[s] = filter_angles(sp, lt, h.x[pr.x], h.y[pr.x], h.z[pr.x], h.x[pr.y], h.y[pr.y],
-                                  h.z[pr.y], h.x[cd.y], h.y[cd.y], h.z[cd.y]);
+          keep = filter_phi(sp, lt, h.x[pr.x], h.y[pr.x], h.x[pr.y], h.y[pr.y], h.x[cd.y], h.y[cd.y]);
         }
       }
-      const uint64_t m = __ballot(keep[s]);
-      if (keep[s]) s_queue[wave][nq + lanes_below(m)] = (uint8_t)(s * 64u + lane);
+      const uint64_t m = __ballot(keep);
+      if (keep) q1[nq + lanes_below(m)] = (uint8_t)(s * 64u + lane);
       nq += (uint32_t)__popcll(m);
     }
     wave_lds_sync();
-    for (uint32_t j = lane; j < nq; j += 64u) {
-      const uint32_t e = s_queue[wave][j];
+    // stage 2: theta-ratio cut on the phi survivors
+    nq = filter_stage(q1, nq, q2, [&](uint32_t e) {
       const uint32_t c = c0 + e;
       const uint2 cd = cands[c];
       const uint2 pr = pairs[cd.x];
       const uint32_t lt = segment_of(offs, K + 1, c) % P;
-      s_pass[wave][e] = filter_circle(sp, lt, h.x[pr.x], h.y[pr.x], h.x[pr.y], h.y[pr.y], h.x[cd.y],
-                                      h.y[cd.y]) ? 1 : 0;
+      return filter_theta(sp, lt, h.x[pr.x], h.y[pr.x], h.z[pr.x], h.x[pr.y], h.y[pr.y], h.z[pr.y],
+                          h.x[cd.y], h.y[cd.y], h.z[cd.y]);
+    });
+    // stage 3: circle fit on the survivors of both angle cuts
+    for (uint32_t j = lane; j < nq; j += 64u) {
+      const uint32_t e = q2[j];
+      const uint32_t c = c0 + e;
+      const uint2 cd = cands[c];
+      const uint2 pr = pairs[cd.x];
+      const uint32_t lt = segment_of(offs, K + 1, c) % P;
+      pass[e] = filter_circle(sp, lt, h.x[pr.x], h.y[pr.x], h.x[pr.y], h.y[pr.y], h.x[cd.y],
+                              h.y[cd.y]) ? 1 : 0;
     }
     wave_lds_sync();
 #pragma unroll
     for (uint32_t s = 0; s < kFilterWords; ++s) {
-      const bool k = keep[s] && s_pass[wave][s * 64u + lane];
-      const uint64_t m = __ballot(k);
+      const uint64_t m = __ballot(pass[s * 64u + lane] != 0);
       if (lane == s) {
         mask[task * kFilterWords + s] = m;
         counts[task * kFilterWords + s] = (uint32_t)__popcll(m);
       }
     }
-    wave_lds_sync();  // s_queue / s_pass are reused by the wave's next task
+    wave_lds_sync();  // the queues and pass flags are reused by the wave's next task
   }
 }
 
