@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TRAX_ABI_VERSION 2
+#define TRAX_ABI_VERSION 3
 
 #define TRAX_OK 0
 #define TRAX_ERR_INVALID (-1)  /* bad argument or configuration */
@@ -107,7 +107,6 @@ typedef struct {
   uint32_t *work_offsets;  /* >= max_events * n_triplets + 1 */
   uint32_t *pair_counts;   /* >= max_hits + 1 */
   uint32_t *cand_counts;   /* >= pair_capacity + 1 */
-  uint32_t *grid_counters; /* >= max_events * n_layers * cells, or NULL */
   uint32_t max_events, max_hits;
   uint64_t pair_capacity;
   uint64_t cand_capacity;
@@ -138,7 +137,7 @@ typedef struct {
 
 /* Launch shape of the kernels, per context (set by the autotuner). */
 typedef struct {
-  uint32_t grid_waves;    /* waves per grid-build block: 1, 2, 4 or 8 (default 8) */
+  uint32_t grid_waves;    /* waves per grid-sort block: 1, 2, 4 or 8 (default 8) */
   uint32_t pairs_block;   /* threads per block, two-phase pair kernels: 64..512 (default 256) */
   uint32_t predict_block; /* threads per block, two-phase prediction kernels (default 256) */
   uint32_t scan_tile;     /* elements per single-pass scan tile: 4096, 8192 or 16384 (default 4096) */
@@ -161,7 +160,7 @@ int trax_ctx_flags(trax_ctx *ctx, void *stream, uint32_t *flags_out, int clear);
  * inside a stream capture are not timed); trax_ctx_kernel_time waits for
  * them and returns the summed device time and the launch count, then
  * resets.  id 0 turns the timer off. */
-#define TRAX_KERNEL_GRID_BUILD 1      /* counting sort into the grid */
+#define TRAX_KERNEL_GRID_BUILD 1      /* grid build: segment sort + grid fill */
 #define TRAX_KERNEL_PAIRS 2           /* single-pass pair generation */
 #define TRAX_KERNEL_PREDICT_WALK 3    /* prediction window walk: counts + stash slabs */
 #define TRAX_KERNEL_PREDICT_PLACE 4   /* prediction slabs -> final candidate slots */
@@ -177,8 +176,7 @@ float trax_min_radius_curvature(float min_pt);
 /* Carves one device allocation at `base` into a workspace for batches of at
  * most max_events events, max_hits hits, pair_capacity pairs and
  * cand_capacity candidates, and returns the bytes it needs (call with
- * base = NULL to size it).  The grid counters are only laid out when
- * (n_z+1)(n_phi+1) cells exceed 64 KiB of LDS. */
+ * base = NULL to size it). */
 uint64_t trax_workspace_layout(const trax_seed_config *cfg, uint32_t max_events, uint32_t max_hits,
                                uint64_t pair_capacity, uint64_t cand_capacity, void *base,
                                trax_workspace *ws);
@@ -201,10 +199,12 @@ int trax_prefix_sum_dev(trax_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, ui
  * hit columns (+ orig_index, phi, zphi) to
  * `sorted` and the cell starts (n_events * n_layers * (n_z+1)(n_phi+1)
  * uint32, global sorted indices) to d_grid.  Within a cell hits keep their
- * input order. */
+ * input order.  Needs no workspace: segments of more than 8192 hits are
+ * sorted in place in the det_id/hit_id/evt/layer output columns before
+ * those are written. */
 int trax_grid_build(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
                     const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,
-                    const trax_workspace *ws, void *stream);
+                    void *stream);
 
 /* ------------------------------------------------------------------ pairs */
 /* PairGeneratorBeamspot::run(...), PairGeneratorBeamspot.h:40-41, in two

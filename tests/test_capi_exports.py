@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_only_entry_points():
     lib = runtime.lib()
-    assert lib.trax_abi_version() == 2
+    assert lib.trax_abi_version() == 3
     for pt in (0.3, 0.5, 1.0):
         assert np.float32(lib.trax_min_radius_curvature(pt)) == ev.min_radius_curvature(pt)
     # 1 GeV at Bz = 3.8112 T: 87.52 cm (TripletConfiguration.cpp:145-152)
@@ -65,13 +65,13 @@ def test_workspace_layout_sizes():
              (words + 1) * 4]
     for (p, sz), q in zip(zip(ptrs, sizes), ptrs[1:] + [base + n]):
         assert p + sz <= q
-    assert not ws.grid_counters  # 65 x 129 cells fit in LDS
+    # the grid build needs no workspace, so the binning does not change it
     big = ev.config_workload(2, 4).seed_config
     big.n_z, big.n_phi = 128, 256
     cfgb = runtime.seed_config_c(big)
     nb = lib.trax_workspace_layout(ctypes.byref(cfgb), 4, 10000, 50000, 100000, ctypes.c_void_p(base),
                                    ctypes.byref(ws))
-    assert ws.grid_counters and nb >= n + 4 * 4 * 129 * 257 * 4
+    assert nb == n
 
 
 def test_context_requires_a_device():

@@ -332,7 +332,6 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
                                           uint32_t max_hits, uint64_t pair_capacity,
                                           uint64_t cand_capacity, void* base, trax_workspace* ws) {
   if (!cfg) return 0;
-  const uint64_t cells = (uint64_t)(cfg->n_z + 1) * (cfg->n_phi + 1);
   const uint64_t words = trax::filter_mask_words(cand_capacity);
   uint64_t off = 0;
   const uint64_t o_work = off;
@@ -348,18 +347,11 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
   off = align_up(off + words * 8);
   const uint64_t o_fc = off;
   off = align_up(off + (words + 1) * 4);
-  uint64_t o_gc = 0;
-  const bool need_gc = cells * 4 > 65536;
-  if (need_gc) {
-    o_gc = off;
-    off = align_up(off + (uint64_t)max_events * cfg->n_layers * cells * 4);
-  }
   if (ws) {
     char* b = static_cast<char*>(base);
     ws->work_offsets = b ? reinterpret_cast<uint32_t*>(b + o_work) : nullptr;
     ws->pair_counts = b ? reinterpret_cast<uint32_t*>(b + o_pc) : nullptr;
     ws->cand_counts = b ? reinterpret_cast<uint32_t*>(b + o_cc) : nullptr;
-    ws->grid_counters = (b && need_gc) ? reinterpret_cast<uint32_t*>(b + o_gc) : nullptr;
     ws->max_events = max_events;
     ws->max_hits = max_hits;
     ws->pair_capacity = pair_capacity;
@@ -395,7 +387,7 @@ extern "C" int trax_prefix_sum(trax_ctx* ctx, uint32_t* d_data, uint64_t n, void
 // --------------------------------------------------------------- grid build
 extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const trax_hits* in,
                                const trax_layout* layout, trax_hits* sorted, uint32_t* d_grid,
-                               const trax_workspace* ws, void* stream) {
+                               void* stream) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   if (!layout) return fail(ctx, TRAX_ERR_INVALID, "null layout");
@@ -424,15 +416,8 @@ extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const
   gl.n_hits = layout->n_hits;
   gl.grid = d_grid;
   gl.flags = ctx->flags;
-  gl.counters = nullptr;
-  gl.use_lds = (uint64_t)sp.cells * 4u <= ctx->lds_max;
   gl.waves = (int)ctx->launch.grid_waves;
   gl.lds_max = ctx->lds_max;
-  if (!gl.use_lds) {
-    if (!ws || !ws->grid_counters || ws->max_events < layout->n_events)
-      return fail(ctx, TRAX_ERR_INVALID, "grid too large for LDS and no workspace grid counters");
-    gl.counters = ws->grid_counters;
-  }
   TRAX_TIMED(ctx, TRAX_KERNEL_GRID_BUILD, static_cast<hipStream_t>(stream), trax::launch_grid_build(sp, gl, static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
@@ -657,7 +642,7 @@ extern "C" int trax_triplet_filter(trax_ctx* ctx, const trax_seed_config* cfg, u
 extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const trax_hits* in,
                                const trax_layout* layout, trax_hits* sorted, uint32_t* d_grid,
                                const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
-  int rc = trax_grid_build(ctx, cfg, in, layout, sorted, d_grid, ws, stream);
+  int rc = trax_grid_build(ctx, cfg, in, layout, sorted, d_grid, stream);
   if (rc) return rc;
   const uint32_t E = layout->n_events;
   if ((rc = trax_pairs_beamspot(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;

@@ -273,16 +273,22 @@ __device__ __forceinline__ bool scan_range(uint32_t a, uint32_t b, const float2*
   return false;
 }
 
-// Grid cell of a hit, GridBuilder.h:80-84 (clamped to the padding row/column).
-__device__ __forceinline__ uint32_t grid_cell(const SeedParams& sp, float x, float y, float z,
-                                              float* phi_out) {
+// Grid cell of a hit with phi = atan2(y, x) already known, GridBuilder.h:80-84
+// (clamped to the padding row/column).
+__device__ __forceinline__ uint32_t grid_cell_zphi(const SeedParams& sp, float z, float phi) {
   uint32_t zs = trax_f2u(__builtin_floorf((z - sp.min_z) / sp.sector_size_z));
-  float phi = trax_atan2f(y, x);
   uint32_t ps = trax_f2u(__builtin_floorf((phi - sp.min_phi) / sp.sector_size_phi));
   zs = zs > sp.n_z ? sp.n_z : zs;
   ps = ps > sp.n_phi ? sp.n_phi : ps;
-  *phi_out = phi;
   return zs * (sp.n_phi + 1) + ps;
+}
+
+// Grid cell of a hit, GridBuilder.h:80-84; phi = atan2(y, x) goes to phi_out.
+__device__ __forceinline__ uint32_t grid_cell(const SeedParams& sp, float x, float y, float z,
+                                              float* phi_out) {
+  const float phi = trax_atan2f(y, x);
+  *phi_out = phi;
+  return grid_cell_zphi(sp, z, phi);
 }
 
 }  // namespace trax

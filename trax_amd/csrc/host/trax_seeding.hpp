@@ -312,7 +312,7 @@ class GridBuilder : public KernelWrapper {
     ws.ensure(cfg, grid.config.nEvents, (uint32_t)hits.size(), 1);
     trax_hits in = hits.input(), out = hits.sorted();
     trax_layout lay{grid.config.nEvents, (uint32_t)hits.size(), es.dOffset.get(), ls.dNHits.get(), ls.dOffset.get()};
-    ctx.check(trax_grid_build(ctx.ctx(), &cfg, &in, &lay, &out, grid.boundary.get(), ws.get(), ctx.stream()),
+    ctx.check(trax_grid_build(ctx.ctx(), &cfg, &in, &lay, &out, grid.boundary.get(), ctx.stream()),
               "GridBuilder::run");
   }
 

@@ -34,11 +34,9 @@ struct GridLaunch {
   const uint32_t *event_offset, *layer_nhits, *layer_offset;
   uint32_t n_hits;
   uint32_t* grid;
-  uint32_t* counters;
   uint32_t* flags;
-  bool use_lds;
-  size_t lds_max;  // dynamic LDS a block may use
-  int waves;       // waves per block: 1, 2, 4 or 8
+  size_t lds_max;  // LDS a sort block may use
+  int waves;       // waves per sort block: 1, 2, 4 or 8
 };
 hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream);
 hipError_t grid_build_set_lds_limit(size_t bytes);

@@ -64,7 +64,7 @@ class LayoutC(C.Structure):
 
 class WorkspaceC(C.Structure):
     _fields_ = [("work_offsets", _vp), ("pair_counts", _vp), ("cand_counts", _vp),
-                ("grid_counters", _vp), ("max_events", C.c_uint32), ("max_hits", C.c_uint32),
+                ("max_events", C.c_uint32), ("max_hits", C.c_uint32),
                 ("pair_capacity", C.c_uint64), ("cand_capacity", C.c_uint64),
                 ("cand_stash", _vp), ("filter_mask", _vp), ("filter_counts", _vp)]
 
@@ -99,7 +99,7 @@ SIGNATURES = {
     "trax_prefix_sum": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),
     "trax_prefix_sum_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint64, _vp, C.c_uint32, _vp]),
     "trax_grid_build": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
-                                  _P(WorkspaceC), _vp]),
+                                  _vp]),
     "trax_pairs_count": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp, _P(WorkspaceC),
                                    _P(OutputsC), _vp]),
     "trax_pairs_store": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp, _P(WorkspaceC),
@@ -440,8 +440,8 @@ def grid_build(ctx: Context, bufs: ChainBuffers, db: DeviceBatch, stream=None) -
     """GridBuilder::run (GridBuilder.h:42) on the device batch."""
     hin, lay, hs = db.hits_c(), db.layout_c(), bufs.sorted_c()
     ctx.check(ctx.lib.trax_grid_build(ctx.h, C.byref(bufs.cfg), C.byref(hin), C.byref(lay),
-                                      C.byref(hs), _vp(_ptr(bufs.grid)), C.byref(bufs.ws),
-                                      stream_ptr(stream)), "trax_grid_build")
+                                      C.byref(hs), _vp(_ptr(bufs.grid)), stream_ptr(stream)),
+              "trax_grid_build")
 
 
 def _stage(ctx, fn, name, bufs, n_events, with_grid=True, stream=None):
