@@ -8,29 +8,36 @@
 //
 // Here two kernels, no host read and no global scan:
 //
-// k_grid_sort, one block per (event, layer) segment: computes every hit's
-// cell, sorts the segment's hits by cell with a stable LSD radix sort (8-bit
-// digits; ranks from 64-lane ballot matches and per-(digit, wave) counters,
-// waves owning consecutive input ranges, so equal cells keep input order),
-// and writes the sorted columns with coalesced stores (the input columns of
-// a segment are gathered from L2).  Segments of up to 8192 hits sort packed
-// (cell << 13 | index) keys in LDS; larger segments, or grids of more than
-// 2^19 cells, sort (cell, index) pairs in place in four of the output
-// columns before those are overwritten.  The block also writes the grid
-// entry of the first cell of every 4096-cell chunk of its segment: the
-// global sorted index of the first hit whose cell is not below that cell.
+// k_grid_cells, elementwise over the batch: phi = atan2(y, x) and the cell of
+// every hit, parked in input order in the phi and orig_index output columns.
+// The transcendental and division work is spread over the whole device
+// instead of over one block per segment.
 //
-// k_grid_fill, one block per 4096-cell chunk of a segment: the hits of the
-// chunk are the sorted range between its first-cell entry and the next
-// chunk's; their cells (recomputed from the sorted z and phi) go into an LDS
-// histogram whose exclusive scan, offset by the range start, gives every
-// cell's start.  Most cells are empty and the grid is most of the bytes this
-// stage writes, so the fill runs over thousands of blocks with coalesced
-// stores.
+// k_grid_sort, one block per (event, layer) segment, everything in LDS:
+// * the hits' cells are counted over the segment's occupied cell range
+//   [cmin, cmax] (a few thousand cells for a barrel layer); the exclusive
+//   scan of the counts is the segment's grid over that range, and below /
+//   above it the grid is the segment base / base + hit count, so the block
+//   writes the whole grid of its segment with coalesced stores;
+// * each hit takes a slot of its cell (LDS atomics) and its final position is
+//   the cell start plus the number of same-cell hits with a smaller input
+//   index, so equal cells keep input order;
+// * a cell holding more than 32 hits, or a range wider than the LDS buffer,
+//   switches to a stable LSD radix sort of packed ((cell - cmin) << 13 |
+//   index) keys (5-bit digits; ranks from 64-lane ballot matches and
+//   per-(digit, wave) counters over consecutive input ranges) and the grid
+//   from binary searches in the sorted keys;
+// * the columns are permuted through LDS: each column is read coalesced in
+//   input order (the next pair is in flight while the current one is
+//   stored) and written coalesced in sorted order.
+// Segments of more than 8192 hits sort (cell, index) pairs with the same
+// radix passes in place in four of the output columns before those are
+// overwritten.
 //
 // A segment's global base is its hit offset: hits are stored event-major and
 // layer-ascending, so the cells of all earlier segments hold exactly the hits
-// before this segment (the reference's scanned grid).
+// before this segment (the reference's scanned grid).  A layout whose
+// segments are not contiguous from hit 0 raises TRAX_FLAG_LAYOUT.
 #include "common.h"
 #include "kernels.h"
 
@@ -44,7 +51,8 @@ struct GridArgs {
   const uint32_t* __restrict__ idet;
   const uint32_t* __restrict__ ihid;
   const uint32_t* __restrict__ ievt;
-  // outputs; odet/ohid/oevt/olayer double as sort scratch for large segments
+  // outputs; ophi/oorig hold (phi, cell) in input order between the cell and
+  // sort kernels, odet/ohid/oevt/olayer are sort scratch for large segments
   float* ox;
   float* oy;
   float* oz;
@@ -63,21 +71,37 @@ struct GridArgs {
   uint32_t* flags;
 };
 
-constexpr uint32_t kGridChunk = 4096;  // cells per fill block
 constexpr uint32_t kKeyIdxBits = 13;   // index bits of a packed LDS key
 constexpr uint32_t kLdsSortMax = 1u << kKeyIdxBits;
-constexpr uint32_t kDigitBits = 8;
+constexpr uint32_t kDigitBits = 5;
 constexpr uint32_t kDigits = 1u << kDigitBits;
-constexpr uint32_t kFillThreads = 256;
+constexpr uint32_t kCellThreads = 256;
+constexpr uint32_t kOccupancyMax = 32;  // counting sort: most hits in one cell
 
-// Hit range of segment `seg` (clamped, with a flag, if the layout does not
-// fit the batch).
+__device__ __forceinline__ void segment_range(const GridArgs& a, uint32_t seg, uint32_t L,
+                                              uint32_t* base, uint32_t* nh) {
+  *base = a.event_offset[seg / L] + a.layer_offset[seg];
+  *nh = a.layer_nhits[seg];
+}
+
+// Hit range of segment `seg`, clamped to the batch.  With `check`, thread 0
+// raises TRAX_FLAG_LAYOUT unless the segment starts where the previous one
+// ends (segment 0 at hit 0) and fits the batch.
 __device__ __forceinline__ void segment_extent(const GridArgs& a, uint32_t seg, uint32_t L,
-                                               uint32_t* base_out, uint32_t* nh_out, bool raise) {
-  uint32_t base = a.event_offset[seg / L] + a.layer_offset[seg];
-  uint32_t nh = a.layer_nhits[seg];
-  if (base > a.n_hits || nh > a.n_hits - base) {
-    if (raise && threadIdx.x == 0) atomicOr(a.flags, TRAX_FLAG_LAYOUT);
+                                               uint32_t* base_out, uint32_t* nh_out, bool check) {
+  uint32_t base, nh;
+  segment_range(a, seg, L, &base, &nh);
+  const bool fits = base <= a.n_hits && nh <= a.n_hits - base;
+  if (check && threadIdx.x == 0) {
+    uint32_t pend = 0;
+    if (seg > 0) {
+      uint32_t pb, pn;
+      segment_range(a, seg - 1, L, &pb, &pn);
+      pend = pb + pn;
+    }
+    if (!fits || base != pend) atomicOr(a.flags, TRAX_FLAG_LAYOUT);
+  }
+  if (!fits) {
     base = base > a.n_hits ? a.n_hits : base;
     nh = 0;
   }
@@ -118,7 +142,7 @@ __device__ __forceinline__ void block_exclusive_scan_cells(uint32_t* cnt, uint32
   }
 }
 
-// Lanes of this wave holding the same 8-bit digit (among `valid` lanes).
+// Lanes of this wave holding the same digit (among `valid` lanes).
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
   uint64_t m = __ballot(valid);
 #pragma unroll
@@ -131,9 +155,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
 }
 
 // One stable counting pass of the radix sort over elements 0..n-1.
-// load(i) -> element, digit(e) -> 0..255, store(pos, e).  Wave w owns the
-// consecutive input range [w*S, (w+1)*S); per-(digit, wave) counters laid
-// out digit-major make the output order (digit, wave, input order).
+// load(i) -> element, digit(e) -> 0..kDigits-1, store(pos, e).  Wave w owns
+// the consecutive input range [w*S, (w+1)*S); per-(digit, wave) counters
+// laid out digit-major make the output order (digit, wave, input order).
 template <int NW, typename Load, typename Digit, typename Store>
 __device__ __forceinline__ void radix_pass(uint32_t n, Load load, Digit digit, Store store,
                                            uint32_t* s_cnt, uint32_t* s_wave) {
@@ -143,13 +167,7 @@ __device__ __forceinline__ void radix_pass(uint32_t n, Load load, Digit digit, S
   const uint32_t hi = lo + S < n ? lo + S : n;
   for (uint32_t c = threadIdx.x; c < kDigits * NW; c += 64u * NW) s_cnt[c] = 0u;
   __syncthreads();
-  for (uint32_t i0 = lo; i0 < hi; i0 += 64u) {
-    const uint32_t i = i0 + lane;
-    const bool valid = i < hi;
-    const uint32_t d = valid ? digit(load(i)) : 0u;
-    const uint64_t m = match_digit(d, valid);
-    if (valid && lanes_below(m) == 0u) s_cnt[d * NW + wave] += (uint32_t)__popcll(m);
-  }
+  for (uint32_t i = lo + lane; i < hi; i += 64u) atomicAdd(&s_cnt[digit(load(i)) * NW + wave], 1u);
   __syncthreads();
   block_exclusive_scan_cells<NW>(s_cnt, kDigits * NW, s_wave);
   for (uint32_t i0 = lo; i0 < hi; i0 += 64u) {
@@ -169,95 +187,277 @@ __device__ __forceinline__ void radix_pass(uint32_t n, Load load, Digit digit, S
   __syncthreads();
 }
 
-// Grid entries of the chunk-start cells k*kGridChunk (k < nch) that the
-// sorted hit at position p opens: those with prev_cell < k*kGridChunk <= cell
-// (prev_cell = -1 for p == 0).
-__device__ __forceinline__ void chunk_starts(uint32_t* g, uint32_t nch, int64_t prev_cell,
-                                             uint32_t cell, uint32_t value) {
-  const uint32_t k_lo = prev_cell < 0 ? 0u : (uint32_t)(prev_cell / kGridChunk) + 1u;
-  const uint32_t k_hi = cell / kGridChunk;  // inclusive
-  for (uint32_t k = k_lo; k <= k_hi && k < nch; ++k) g[(uint64_t)k * kGridChunk] = value;
+// Elementwise: (phi, cell) of every hit of the batch, in input order.
+__global__ __launch_bounds__(kCellThreads) void k_grid_cells(SeedParams sp, GridArgs a) {
+  const uint32_t n = a.n_hits;
+  for (uint32_t i = blockIdx.x * kCellThreads + threadIdx.x; i < n; i += gridDim.x * kCellThreads) {
+    const float phi = trax_atan2f(a.iy[i], a.ix[i]);
+    a.ophi[i] = phi;
+    a.oorig[i] = grid_cell_zphi(sp, a.iz[i], phi);
+  }
 }
 
-__device__ __forceinline__ void write_sorted(const GridArgs& a, uint32_t dst, uint32_t src) {
-  const float x = a.ix[src], y = a.iy[src], z = a.iz[src];
-  const float phi = trax_atan2f(y, x);
-  a.ox[dst] = x;
-  a.oy[dst] = y;
-  a.oz[dst] = z;
-  a.olayer[dst] = a.ilayer[src];
-  a.odet[dst] = a.idet[src];
-  a.ohid[dst] = a.ihid[src];
-  a.oevt[dst] = a.ievt[src];
-  a.oorig[dst] = src;
-  a.ophi[dst] = phi;
-  a.ozphi[dst] = make_float2(z, phi);
+// Block-wide min of lo and max of hi (every thread gets both).
+template <int NW>
+__device__ __forceinline__ void block_min_max(uint32_t lo, uint32_t hi, uint32_t* s_wave,
+                                              uint32_t* out_lo, uint32_t* out_hi) {
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint32_t l = __shfl_xor(lo, d, 64), h = __shfl_xor(hi, d, 64);
+    lo = l < lo ? l : lo;
+    hi = h > hi ? h : hi;
+  }
+  if (lane == 0) {
+    s_wave[wave] = lo;
+    s_wave[NW + wave] = hi;
+  }
+  __syncthreads();
+  lo = s_wave[0];
+  hi = s_wave[NW];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) {
+    lo = s_wave[w] < lo ? s_wave[w] : lo;
+    hi = s_wave[NW + w] > hi ? s_wave[NW + w] : hi;
+  }
+  __syncthreads();
+  *out_lo = lo;
+  *out_hi = hi;
 }
 
+// Number of keys in S[0..n) below t (S sorted ascending).
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* S, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, len = n;
+  while (len > 0) {
+    const uint32_t half = len >> 1;
+    if (S[lo + half] < t) {
+      lo += half + 1u;
+      len -= half + 1u;
+    } else {
+      len = half;
+    }
+  }
+  return lo;
+}
+
+// Grid entries of a segment outside its occupied cell range [cmin, cmax]:
+// the segment base below it, base + nh above it (coalesced stores).
+template <uint32_t BLOCK>
+__device__ __forceinline__ void grid_constant_parts(uint32_t* g, uint32_t C, uint32_t cmin,
+                                                    uint32_t cmax, uint32_t base, uint32_t nh) {
+  for (uint32_t c = threadIdx.x; c < cmin && c < C; c += BLOCK) g[c] = base;
+  for (uint32_t c = cmax + 1u + threadIdx.x; c < C; c += BLOCK) g[c] = base + nh;
+}
+
+// In-block exclusive scan of cnt[0..n) that also returns the largest count.
+template <int NW>
+__device__ __forceinline__ uint32_t block_scan_occupancy(uint32_t* cnt, uint32_t n, uint32_t* s_wave) {
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  uint32_t carry = 0, occ = 0;
+  for (uint32_t base = 0; base < n; base += 256u * NW) {
+    const uint32_t c0 = base + tid * 4u;
+    const uint32_t v0 = c0 < n ? cnt[c0] : 0u;
+    const uint32_t v1 = c0 + 1 < n ? cnt[c0 + 1] : 0u;
+    const uint32_t v2 = c0 + 2 < n ? cnt[c0 + 2] : 0u;
+    const uint32_t v3 = c0 + 3 < n ? cnt[c0 + 3] : 0u;
+    const uint32_t m01 = v0 > v1 ? v0 : v1, m23 = v2 > v3 ? v2 : v3;
+    const uint32_t m = m01 > m23 ? m01 : m23;
+    occ = m > occ ? m : occ;
+    const uint32_t t = v0 + v1 + v2 + v3;
+    const uint32_t incl = wave_inclusive_scan(t);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t c = s_wave[w];
+      wp += (w < (int)wave) ? c : 0u;
+      tot += c;
+    }
+    const uint32_t e = carry + wp + incl - t;
+    if (c0 < n) cnt[c0] = e;
+    if (c0 + 1 < n) cnt[c0 + 1] = e + v0;
+    if (c0 + 2 < n) cnt[c0 + 2] = e + v0 + v1;
+    if (c0 + 3 < n) cnt[c0 + 3] = e + v0 + v1 + v2;
+    carry += tot;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint32_t o = __shfl_xor(occ, d, 64);
+    occ = o > occ ? o : occ;
+  }
+  if (lane == 0) s_wave[NW + wave] = occ;
+  __syncthreads();
+  occ = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) occ = s_wave[NW + w] > occ ? s_wave[NW + w] : occ;
+  __syncthreads();
+  return occ;
+}
+
+// Per-hit state lives in LDS, so the kernel needs few registers and two
+// blocks fit on a CU: every segment of a 64-event batch is resident from the
+// start.
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a, uint32_t lds_cap) {
   constexpr uint32_t BLOCK = 64u * NW;
-  extern __shared__ uint32_t s_keys[];  // 2 * lds_cap packed keys (ping-pong)
+  extern __shared__ uint32_t s_lds[];  // R0, R1: lds_cap words each; R2: lds_cap uint16
   __shared__ uint32_t s_cnt[kDigits * NW];
-  __shared__ uint32_t s_wave[NW];
-  __shared__ uint32_t s_carry;
+  __shared__ uint32_t s_wave[2 * NW];
   const uint32_t tid = threadIdx.x;
   const uint32_t C = sp.cells, L = sp.n_layers;
   const uint32_t nseg = sp.n_events * L;
-  const uint32_t nch = (C + kGridChunk - 1u) / kGridChunk;
-  const uint32_t passes = (sp.cell_bits + kDigitBits - 1u) / kDigitBits;
-  const bool packed_ok = sp.cell_bits + kKeyIdxBits <= 32u;
+  const uint32_t cap = lds_cap;
+  uint32_t* R0 = s_lds;
+  uint32_t* R1 = s_lds + lds_cap;
+  uint16_t* R2 = reinterpret_cast<uint16_t*>(s_lds + 2u * lds_cap);
   for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
     uint32_t base, nh;
     segment_extent(a, seg, L, &base, &nh, true);
     uint32_t* g = a.grid + (uint64_t)seg * C;
-    if (packed_ok && nh <= lds_cap) {
-      uint32_t* src = s_keys;
-      uint32_t* dst = s_keys + lds_cap;
+    if (nh <= cap) {
+      // cells in input order into R1; the segment's occupied cell range
+      uint32_t cmin = 0xFFFFFFFFu, cmax = 0u;
       for (uint32_t i = tid; i < nh; i += BLOCK) {
-        float phi;
-        const uint32_t cell = grid_cell(sp, a.ix[base + i], a.iy[base + i], a.iz[base + i], &phi);
-        src[i] = (cell << kKeyIdxBits) | i;
+        const uint32_t c = a.oorig[base + i];
+        cmin = c < cmin ? c : cmin;
+        cmax = c > cmax ? c : cmax;
+        R1[i] = c;
       }
-      __syncthreads();
-      for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t shift = kKeyIdxBits + p * kDigitBits;
-        const uint32_t* s = src;
-        uint32_t* d = dst;
-        radix_pass<NW>(
-            nh, [s](uint32_t i) { return s[i]; },
-            [shift](uint32_t e) { return (e >> shift) & (kDigits - 1u); },
-            [d](uint32_t pos, uint32_t e) { d[pos] = e; }, s_cnt, s_wave);
-        uint32_t* t = src;
-        src = dst;
-        dst = t;
+      block_min_max<NW>(cmin, cmax, s_wave, &cmin, &cmax);
+      if (nh == 0) {
+        for (uint32_t c = tid; c < C; c += BLOCK) g[c] = base;
+        continue;
       }
-      for (uint32_t p = tid; p < nh; p += BLOCK) {
-        const uint32_t v = src[p];
-        const uint32_t cell = v >> kKeyIdxBits;
-        const int64_t prev = p == 0 ? -1 : (int64_t)(src[p - 1] >> kKeyIdxBits);
-        write_sorted(a, base + p, base + (v & (kLdsSortMax - 1u)));
-        chunk_starts(g, nch, prev, cell, base + p);
+      const uint32_t span = cmax - cmin + 1u;  // cells <= 2^24
+      uint32_t span_bits = 0;
+      while (((span - 1u) >> span_bits) != 0u) ++span_bits;
+      if (span_bits + kKeyIdxBits <= 32u) {  // block-uniform
+        // most of the grid is outside the occupied range: those stores drain
+        // while the sort works in LDS
+        grid_constant_parts<BLOCK>(g, C, cmin, cmax, base, nh);
+        const uint32_t* S = nullptr;  // sorted packed keys ((cell - cmin) << 13 | index)
+        if (span + nh <= lds_cap) {
+          // Counting sort over the occupied range: R0 holds the cell counts
+          // (then starts, then ends) followed by the slot -> hit map.
+          uint32_t* slot_hit = R0 + span;
+          for (uint32_t x = tid; x < span; x += BLOCK) R0[x] = 0u;
+          __syncthreads();
+          for (uint32_t i = tid; i < nh; i += BLOCK) atomicAdd(&R0[R1[i] - cmin], 1u);
+          __syncthreads();
+          const uint32_t occ = block_scan_occupancy<NW>(R0, span, s_wave);
+          if (occ <= kOccupancyMax) {  // block-uniform
+            // the scanned counts are the grid over the occupied range
+            for (uint32_t x = tid; x < span; x += BLOCK) g[cmin + x] = base + R0[x];
+            __syncthreads();
+            // each hit takes a slot of its cell: R1[i] = (cell - cmin) << 13 | slot
+            for (uint32_t i = tid; i < nh; i += BLOCK) {
+              const uint32_t x = R1[i] - cmin;
+              const uint32_t slot = atomicAdd(&R0[x], 1u);
+              slot_hit[slot] = i;
+              R1[i] = (x << kKeyIdxBits) | slot;
+            }
+            __syncthreads();
+            // final position: cell start + same-cell hits with a smaller
+            // index (R0[x] is now the end of cell x, R0[x - 1] its start)
+            for (uint32_t i = tid; i < nh; i += BLOCK) {
+              const uint32_t x = R1[i] >> kKeyIdxBits;
+              const uint32_t st = x ? R0[x - 1u] : 0u, en = R0[x];
+              uint32_t rank = 0;
+              for (uint32_t t = st; t < en; ++t) rank += slot_hit[t] < i ? 1u : 0u;
+              R1[i] = (x << kKeyIdxBits) | (st + rank);
+            }
+            __syncthreads();
+            for (uint32_t i = tid; i < nh; i += BLOCK) {
+              const uint32_t v = R1[i];
+              R0[v & (kLdsSortMax - 1u)] = (v & ~(kLdsSortMax - 1u)) | i;
+            }
+            __syncthreads();
+            S = R0;
+          }
+        }
+        if (S == nullptr) {
+          // radix sort of the packed keys (wide range or a crowded cell);
+          // R1 still holds the cells
+          for (uint32_t i = tid; i < nh; i += BLOCK) R0[i] = ((R1[i] - cmin) << kKeyIdxBits) | i;
+          __syncthreads();
+          uint32_t* src = R0;
+          uint32_t* dst = R1;
+          const uint32_t passes = (span_bits + kDigitBits - 1u) / kDigitBits;
+          for (uint32_t p = 0; p < passes; ++p) {
+            const uint32_t shift = kKeyIdxBits + p * kDigitBits;
+            const uint32_t* s = src;
+            uint32_t* d = dst;
+            radix_pass<NW>(
+                nh, [s](uint32_t i) { return s[i]; },
+                [shift](uint32_t e) { return (e >> shift) & (kDigits - 1u); },
+                [d](uint32_t pos, uint32_t e) { d[pos] = e; }, s_cnt, s_wave);
+            uint32_t* t = src;
+            src = dst;
+            dst = t;
+          }
+          S = src;
+          for (uint32_t x = tid; x < span; x += BLOCK)
+            g[cmin + x] = base + lower_bound_u32(S, nh, x << kKeyIdxBits);
+        }
+        // Columns in sorted order, two per round: the sorted input indexes
+        // move to the 16-bit region R2, the columns are read coalesced in
+        // input order into R0/R1, gathered by sorted position and stored
+        // coalesced.  Each column is read completely before it is written,
+        // so the input-order phi parked in the phi column is safe.
+        for (uint32_t p = tid; p < nh; p += BLOCK) R2[p] = (uint16_t)(S[p] & (kLdsSortMax - 1u));
+        __syncthreads();
+        const uint32_t* col_a[4] = {reinterpret_cast<const uint32_t*>(a.ix),
+                                    reinterpret_cast<const uint32_t*>(a.iz), a.ilayer, a.ihid};
+        const uint32_t* col_b[4] = {reinterpret_cast<const uint32_t*>(a.iy),
+                                    reinterpret_cast<const uint32_t*>(a.ophi), a.idet, a.ievt};
+        for (uint32_t r = 0; r < 4; ++r) {
+          const uint32_t* ca = col_a[r] + base;
+          const uint32_t* cb = col_b[r] + base;
+          for (uint32_t i = tid; i < nh; i += BLOCK) {
+            R0[i] = ca[i];
+            R1[i] = cb[i];
+          }
+          __syncthreads();
+          for (uint32_t p = tid; p < nh; p += BLOCK) {
+            const uint32_t hit = R2[p];
+            const uint32_t va = R0[hit], vb = R1[hit];
+            const uint32_t dst = base + p;
+            if (r == 0) {
+              a.ox[dst] = __builtin_bit_cast(float, va);
+              a.oy[dst] = __builtin_bit_cast(float, vb);
+            } else if (r == 1) {
+              a.oz[dst] = __builtin_bit_cast(float, va);
+              a.ophi[dst] = __builtin_bit_cast(float, vb);
+              a.ozphi[dst] = make_float2(__builtin_bit_cast(float, va), __builtin_bit_cast(float, vb));
+            } else if (r == 2) {
+              a.olayer[dst] = va;
+              a.odet[dst] = vb;
+            } else {
+              a.ohid[dst] = va;
+              a.oevt[dst] = vb;
+              a.oorig[dst] = base + hit;
+            }
+          }
+          __syncthreads();
+        }
+        continue;
       }
-      if (tid == 0) {
-        const int64_t last = nh == 0 ? -1 : (int64_t)(src[nh - 1] >> kKeyIdxBits);
-        chunk_starts(g, nch, last, (nch - 1u) * kGridChunk, base + nh);
-      }
-      __syncthreads();
-      continue;
     }
-    // Large segment: (cell, index) pairs sorted in place in the output
-    // columns odet/ohid and oevt/olayer, which are written last.
+    // Large segment (or a span too wide for packed keys): (cell, index)
+    // pairs sorted in place in the output columns odet/ohid and oevt/olayer,
+    // which are written last.
     uint32_t* cell_a = a.odet + base;
     uint32_t* idx_a = a.ohid + base;
     uint32_t* cell_b = a.oevt + base;
     uint32_t* idx_b = a.olayer + base;
     for (uint32_t i = tid; i < nh; i += BLOCK) {
-      float phi;
-      cell_a[i] = grid_cell(sp, a.ix[base + i], a.iy[base + i], a.iz[base + i], &phi);
+      cell_a[i] = a.oorig[base + i];
       idx_a[i] = i;
     }
     __syncthreads();
+    const uint32_t passes = (sp.cell_bits + kDigitBits - 1u) / kDigitBits;
     for (uint32_t p = 0; p < passes; ++p) {
       const uint32_t shift = p * kDigitBits;
       const uint32_t *sc = cell_a, *si = idx_a;
@@ -277,70 +477,35 @@ __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a
       idx_a = idx_b;
       idx_b = t;
     }
-    if (tid == 0) s_carry = 0xFFFFFFFFu;
+    // grid: lower bounds in the sorted cells, read before they are
+    // overwritten below
+    for (uint32_t c = tid; c < C; c += BLOCK) g[c] = base + lower_bound_u32(cell_a, nh, c);
     __syncthreads();
     for (uint32_t p0 = 0; p0 < nh; p0 += BLOCK) {
-      // read this round's keys (and the previous round's last cell) before
-      // any thread overwrites the scratch columns
+      // read this round's indexes before any thread overwrites the scratch
+      // columns; phi is recomputed because its input-order copy is
+      // overwritten round by round
       const uint32_t p = p0 + tid;
       const bool valid = p < nh;
-      uint32_t cell = 0, idx = 0;
-      int64_t prev = -1;
+      const uint32_t idx = valid ? idx_a[p] : 0u;
+      __syncthreads();
       if (valid) {
-        cell = cell_a[p];
-        idx = idx_a[p];
-        if (p > 0) prev = tid == 0 ? (int64_t)s_carry : (int64_t)cell_a[p - 1];
+        const uint32_t src = base + idx, dst = base + p;
+        const float x = a.ix[src], y = a.iy[src], z = a.iz[src];
+        const float phi = trax_atan2f(y, x);
+        a.ox[dst] = x;
+        a.oy[dst] = y;
+        a.oz[dst] = z;
+        a.olayer[dst] = a.ilayer[src];
+        a.odet[dst] = a.idet[src];
+        a.ohid[dst] = a.ihid[src];
+        a.oevt[dst] = a.ievt[src];
+        a.oorig[dst] = src;
+        a.ophi[dst] = phi;
+        a.ozphi[dst] = make_float2(z, phi);
       }
       __syncthreads();
-      if (valid && (tid == BLOCK - 1u || p == nh - 1u)) s_carry = cell;
-      if (valid) {
-        write_sorted(a, base + p, base + idx);
-        chunk_starts(g, nch, prev, cell, base + p);
-      }
-      __syncthreads();
     }
-    if (tid == 0) {
-      const int64_t last = nh == 0 ? -1 : (int64_t)s_carry;
-      chunk_starts(g, nch, last, (nch - 1u) * kGridChunk, base + nh);
-    }
-    __syncthreads();
-  }
-}
-
-// One block per (segment, 4096-cell chunk): LDS histogram of the chunk's
-// hits, exclusive scan, coalesced grid stores.
-__global__ __launch_bounds__(kFillThreads) void k_grid_fill(SeedParams sp, GridArgs a) {
-  __shared__ uint32_t s_cnt[kGridChunk];
-  __shared__ uint32_t s_wave[kFillThreads / 64];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t C = sp.cells, L = sp.n_layers;
-  const uint32_t nseg = sp.n_events * L;
-  const uint32_t nch = (C + kGridChunk - 1u) / kGridChunk;
-  const uint64_t nblk = (uint64_t)nseg * nch;
-  for (uint64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
-    const uint32_t seg = (uint32_t)(b / nch), k = (uint32_t)(b % nch);
-    uint32_t base, nh;
-    segment_extent(a, seg, L, &base, &nh, false);
-    const uint32_t c0 = k * kGridChunk;
-    const uint32_t len = C - c0 < kGridChunk ? C - c0 : kGridChunk;
-    uint32_t* g = a.grid + (uint64_t)seg * C + c0;
-    // chunk-start entries written by k_grid_sort (in [base, base + nh])
-    uint32_t lo = g[0] - base;
-    uint32_t hi = k + 1u < nch ? g[kGridChunk] - base : nh;
-    hi = hi > nh ? nh : hi;
-    lo = lo > hi ? hi : lo;
-    for (uint32_t c = tid; c < kGridChunk; c += kFillThreads) s_cnt[c] = 0u;
-    __syncthreads();
-    for (uint32_t j = lo + tid; j < hi; j += kFillThreads) {
-      const float2 zp = a.ozphi[base + j];
-      const uint32_t rel = grid_cell_zphi(sp, zp.x, zp.y) - c0;
-      if (rel < len) atomicAdd(&s_cnt[rel], 1u);
-    }
-    __syncthreads();
-    block_exclusive_scan_cells<kFillThreads / 64>(s_cnt, len, s_wave);
-    const uint32_t start = base + lo;
-    for (uint32_t c = tid; c < len; c += kFillThreads) g[c] = start + s_cnt[c];
-    __syncthreads();
   }
 }
 
@@ -349,6 +514,11 @@ static void grid_sort_launch(uint32_t blocks, size_t lds, uint32_t cap, hipStrea
                              const SeedParams& sp, const GridArgs& a) {
   hipLaunchKernelGGL((k_grid_sort<NW>), dim3(blocks), dim3(64 * NW), lds, stream, sp, a, cap);
 }
+
+// Static LDS of a sort block (counters, wave partials) stays below this; a
+// block uses at most half of a CU's 160 KiB.
+constexpr size_t kSortStaticLds = 2048;
+constexpr size_t kSortLdsPerBlock = 80u * 1024u;
 
 hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream) {
   GridArgs a;
@@ -363,24 +533,28 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
   a.flags = gl.flags;
   const uint32_t nseg = sp.n_events * sp.n_layers;
   if (nseg == 0) return hipSuccess;
-  // packed keys in LDS: two buffers of `cap` keys next to the static
-  // counters (at most 8 KiB + 64 B)
-  const size_t room = gl.lds_max > 8448u ? gl.lds_max - 8448u : 0u;
-  uint32_t cap = (uint32_t)(room / 8u);
+  if (gl.n_hits > 0) {
+    const uint32_t want = (gl.n_hits + kCellThreads - 1u) / kCellThreads;
+    const uint32_t blocks = want < 8192u ? want : 8192u;
+    hipLaunchKernelGGL(k_grid_cells, dim3(blocks), dim3(kCellThreads), 0, stream, sp, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  // 10 bytes of LDS per hit (two word regions, one 16-bit region), sized so
+  // that two blocks share a CU
+  const size_t budget = gl.lds_max < kSortLdsPerBlock ? gl.lds_max : kSortLdsPerBlock;
+  const size_t room = budget > kSortStaticLds ? budget - kSortStaticLds : 0u;
+  uint32_t cap = (uint32_t)(room / 10u);
   cap = cap > kLdsSortMax ? kLdsSortMax : cap & ~63u;
-  const size_t lds = (size_t)cap * 8u;
+  const size_t lds = (size_t)(cap > 0 ? cap : 64u) * 10u;
   const uint32_t blocks = nseg < 65535u ? nseg : 65535u;
   switch (gl.waves) {
     case 1: grid_sort_launch<1>(blocks, lds, cap, stream, sp, a); break;
     case 2: grid_sort_launch<2>(blocks, lds, cap, stream, sp, a); break;
     case 4: grid_sort_launch<4>(blocks, lds, cap, stream, sp, a); break;
+    case 16: grid_sort_launch<16>(blocks, lds, cap, stream, sp, a); break;
     default: grid_sort_launch<8>(blocks, lds, cap, stream, sp, a); break;
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const uint64_t nblk = (uint64_t)nseg * ((sp.cells + kGridChunk - 1u) / kGridChunk);
-  const uint32_t fill_blocks = nblk < (1u << 20) ? (uint32_t)nblk : (1u << 20);
-  hipLaunchKernelGGL(k_grid_fill, dim3(fill_blocks), dim3(kFillThreads), 0, stream, sp, a);
   return hipGetLastError();
 }
 
@@ -389,9 +563,10 @@ hipError_t grid_build_set_lds_limit(size_t bytes) {
   const void* fns[] = {reinterpret_cast<const void*>(&k_grid_sort<1>),
                        reinterpret_cast<const void*>(&k_grid_sort<2>),
                        reinterpret_cast<const void*>(&k_grid_sort<4>),
-                       reinterpret_cast<const void*>(&k_grid_sort<8>)};
-  const size_t dyn = bytes > 8448u ? bytes - 8448u : 0u;
-  const size_t want = (size_t)kLdsSortMax * 8u;
+                       reinterpret_cast<const void*>(&k_grid_sort<8>),
+                       reinterpret_cast<const void*>(&k_grid_sort<16>)};
+  const size_t dyn = bytes > kSortStaticLds ? bytes - kSortStaticLds : 0u;
+  const size_t want = kSortLdsPerBlock - kSortStaticLds;
   for (const void* f : fns) {
     hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)(dyn < want ? dyn : want));
