@@ -10,6 +10,9 @@
            SIMDs' VALU issue slots used (one wave64 VALU instruction per SIMD
            every 4 cycles), the share of lanes active per VALU instruction,
            and the share of wave time spent waiting on dependencies.
+`counters`: per-kernel averages of every counter of one counter run, e.g.
+           `--pmc OccupancyPercent LdsBankConflict LdsUtil SQ_WAVES` (wave
+           occupancy, LDS bank-conflict cycles per LDS access, LDS busy %).
 `traffic`: HBM bytes per launch of each chain kernel from two counter runs
            (`--pmc FETCH_SIZE`, then `--pmc WRITE_SIZE`), each counter scaled
            by what it reads for a copy kernel of known size (torch's int32
@@ -32,7 +35,8 @@ from typing import Dict, List
 
 # Chain kernels by the short names the library's kernel timer uses.
 SHORT = [
-    ("k_grid_build", "grid_build"),
+    ("k_grid_cells", "grid_cells"),
+    ("k_grid_sort", "grid_sort"),
     ("k_pairs_fused", "pairs"),
     ("k_pairs<", "pairs_two_phase"),
     ("k_predict_walk", "predict_walk"),
@@ -125,6 +129,30 @@ def sq(d: str, xcds: int = 8, simds: int = 1024) -> str:
     return "\n".join(out)
 
 
+def counters(d: str, kernels=None) -> str:
+    """Per-kernel average of every counter in one counter-collection run
+    (raw counters and rocprofv3's derived metrics alike), one column each."""
+    rows = _rows(_one(os.path.join(d, "**", "*counter_collection.csv")))
+    per: Dict[str, Dict[str, List[float]]] = defaultdict(lambda: defaultdict(list))
+    names: List[str] = []
+    for r in rows:
+        k = short_name(r["Kernel_Name"])
+        if kernels and k not in kernels:
+            continue
+        n = r["Counter_Name"]
+        if n not in names:
+            names.append(n)
+        per[k][n].append(float(r["Counter_Value"]))
+    out = ["| kernel | " + " | ".join(names) + " |", "|---|" + "---|" * len(names)]
+    for k, cs in per.items():
+        vals = []
+        for n in names:
+            v = cs.get(n)
+            vals.append(f"{sum(v) / len(v):.3g}" if v else "")
+        out.append(f"| {k} | " + " | ".join(vals) + " |")
+    return "\n".join(out)
+
+
 def traffic(fetch_dir: str, write_dir: str, workload: str) -> str:
     fetch = _counter_means(fetch_dir, "FETCH_SIZE")
     write = _counter_means(write_dir, "WRITE_SIZE")
@@ -136,11 +164,17 @@ def traffic(fetch_dir: str, write_dir: str, workload: str) -> str:
     lines = [f"# workload={workload} fetch_scale={f_scale:.4f} write_scale={w_scale:.4f} "
              f"(copy kernel: FETCH_SIZE {fetch['copy_calibration']:.0f} KiB, WRITE_SIZE "
              f"{write['copy_calibration']:.0f} KiB for {kib:.0f} KiB each way)"]
+    total = {}
     for k in sorted(set(fetch) & set(write)):
         fb = fetch[k] * f_scale * 1024.0
         wb = write[k] * w_scale * 1024.0
+        total[k] = fb + wb
         lines.append(f"kernel={k} fetch_kib={fetch[k]:.1f} write_kib={write[k]:.1f} "
                      f"hbm_bytes={fb + wb:.0f}")
+    # the grid stage is two kernels (cells, then the per-segment sort)
+    if "grid_cells" in total and "grid_sort" in total:
+        lines.append(f"kernel=grid_build hbm_bytes={total['grid_cells'] + total['grid_sort']:.0f} "
+                     f"(grid_cells + grid_sort)")
     return "\n".join(lines)
 
 
@@ -153,6 +187,10 @@ def main():
     c = sub.add_parser("sq")
     c.add_argument("dir")
     c.add_argument("--command", default="")
+    d = sub.add_parser("counters")
+    d.add_argument("dir")
+    d.add_argument("--kernels", default="", help="comma-separated short names (default: all)")
+    d.add_argument("--command", default="")
     b = sub.add_parser("traffic")
     b.add_argument("fetch_dir")
     b.add_argument("write_dir")
@@ -165,6 +203,8 @@ def main():
         print(stats(args.dir))
     elif args.what == "sq":
         print(sq(args.dir))
+    elif args.what == "counters":
+        print(counters(args.dir, [k for k in args.kernels.split(",") if k] or None))
     else:
         print(traffic(args.fetch_dir, args.write_dir, args.workload))
 
