@@ -203,7 +203,7 @@ def main():
     st = {}
     E = db.n_events
     st["grid_build"] = time_events(lambda: runtime.grid_build(ctx, bufs, db), 5, torch)
-    st["pairs"] = time_events(lambda: runtime.pairs_beamspot(ctx, bufs, E), 5, torch)
+    st["pairs"] = time_events(lambda: runtime.pair_stage(ctx, bufs, E), 5, torch)
     st["predict"] = time_events(lambda: runtime.triplet_predict(ctx, bufs, E), 5, torch)
     st["filter"] = time_events(lambda: runtime.triplet_filter(ctx, bufs, E), 5, torch)
     log("stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st.items()))

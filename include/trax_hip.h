@@ -73,7 +73,13 @@ typedef struct {
   /* minPt * GEV_C / (Bz * Q) * 100 with Bz = 3.8112 (TripletConfiguration.h:56-70);
    * trax_min_radius_curvature() evaluates it the reference's way. */
   float min_radius_curvature;
+  /* pair generator run by trax_seed_chain: TRAX_PAIRS_BEAMSPOT
+   * (PairGeneratorBeamspot) or TRAX_PAIRS_SECTOR (PairGeneratorSector) */
+  uint32_t pair_generator;
 } trax_seed_config;
+
+#define TRAX_PAIRS_BEAMSPOT 0u
+#define TRAX_PAIRS_SECTOR 1u
 
 /* Hit columns.  Input: x..evt.  Grid-sorted output: all of them, plus
  * orig_index (input index of each sorted hit), phi (atan2(y, x)) and zphi
@@ -225,6 +231,27 @@ int trax_pairs_beamspot(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_e
                         const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
                         trax_seed_outputs *out, void *stream);
 
+/* Sector pairs: PairGeneratorSector::run(...), PairGeneratorSector.h:38-39
+ * (the reference's second PairGenerator, same arguments as the beam-spot
+ * one).  Every inner-layer hit pairs with every outer-layer hit in the grid
+ * cells of its z row +- pair_spread_z and phi cell +- pair_spread_phi (with
+ * the reference's wrap-around and unsigned row arithmetic); no geometric
+ * cut.  Requires 2 * pair_spread_phi <= n_phi.  count: per inner hit counts
+ * + prefix sum (total -> out->totals[0], offsets -> out->pair_offsets);
+ * store: the (inner, outer) records.  trax_pairs_sector runs both (no host
+ * read, graph-capture safe).  An outer hit is paired once per inner hit even
+ * where the reference's window visits it twice (2 * spread + 1 > n_phi); the
+ * reference counts those visits and leaves their slots unwritten. */
+int trax_pairs_sector_count(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                            const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                            trax_seed_outputs *out, void *stream);
+int trax_pairs_sector_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                            const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                            trax_seed_outputs *out, void *stream);
+int trax_pairs_sector(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                      const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
+                      trax_seed_outputs *out, void *stream);
+
 /* ------------------------------------------------------------- prediction */
 /* TripletThetaPhiPredictor::run(...), TripletThetaPhiPredictor.h:43-44, in
  * two phases: count (+ prefix sum; total -> out->totals[1]) and store
@@ -267,8 +294,8 @@ int trax_triplet_filter(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_e
 /* ------------------------------------------------------------ whole chain */
 /* The four stages back to back with no host synchronisation (the stage
  * sequence of buildTriplets, src/triplet_builder/TripletBuilder.cpp:240-263):
- * trax_grid_build, trax_pairs_beamspot, trax_triplet_predict,
- * trax_triplet_filter.  Safe to capture in a hipGraph: no allocation, no
+ * trax_grid_build, trax_pairs_beamspot (or trax_pairs_sector, by
+ * cfg->pair_generator), trax_triplet_predict, trax_triplet_filter.  Safe to capture in a hipGraph: no allocation, no
  * memset, no host read. */
 int trax_seed_chain(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
                     const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,

@@ -40,6 +40,7 @@ class OracleCfg(C.Structure):
         ("pair_spread_z", C.c_uint32 * MAX_TRIPLETS), ("pair_spread_phi", C.c_uint32 * MAX_TRIPLETS),
         ("min_radius_curvature", C.c_float),
         ("threads", C.c_uint32),
+        ("pair_generator", C.c_uint32),
     ]
 
 
@@ -88,6 +89,8 @@ def lib():
         L.oracle_cell_of.restype = C.c_uint32
         L.oracle_pair_window.argtypes = [C.POINTER(OracleCfg), C.c_uint32, C.c_float, C.c_float,
                                          C.c_float, _f32p]
+        L.oracle_sector_window.argtypes = [C.POINTER(OracleCfg), C.c_uint32, C.c_float, C.c_float,
+                                           C.c_float, _u32p]
         L.oracle_predict_window.argtypes = [C.POINTER(OracleCfg), C.c_uint32, C.c_int, _f32p, _f32p,
                                             C.c_float, C.c_float, _f32p]
         L.oracle_filter_eval.argtypes = [C.POINTER(OracleCfg), C.c_uint32, _f32p, _f32p, _f32p, _f32p]
@@ -97,6 +100,9 @@ def lib():
 
 def _p(a, t):
     return a.ctypes.data_as(t)
+
+
+PAIR_GENERATORS = {"beamspot": 0, "sector": 1}  # ORACLE_PAIRS_BEAMSPOT / _SECTOR
 
 
 def make_cfg(sc, n_events: int) -> OracleCfg:
@@ -124,6 +130,7 @@ def make_cfg(sc, n_events: int) -> OracleCfg:
         c.pair_spread_phi[p] = t.pair_spread_phi
     c.min_radius_curvature = float(sc.min_radius_curvature)
     c.threads = sc.threads
+    c.pair_generator = PAIR_GENERATORS[sc.pair_generator]
     return c
 
 
@@ -163,6 +170,7 @@ def run_chain(sc, batch, stop_after: int = 3) -> Dict[str, np.ndarray]:
             out["pairs"] = _arr(res.pairs, 2 * res.n_pairs, np.uint32).reshape(-1, 2)
             out["pair_offsets"] = _arr(res.pair_offsets, nk, np.uint32)
             out["n_pairs_unwritten"] = int(res.n_pairs_unwritten)
+            out["n_pairs_raw_overcount"] = int(res.n_pairs_raw_overcount)
         if stop_after >= 2:
             out["cands"] = _arr(res.cands, 2 * res.n_cands, np.uint32).reshape(-1, 2)
             out["cand_offsets"] = _arr(res.cand_offsets, nk, np.uint32)
@@ -210,6 +218,16 @@ def pair_window(sc, p: int, x: float, y: float, z: float) -> dict:
     out = (C.c_float * 9)()
     lib().oracle_pair_window(C.byref(cfg), p, x, y, z, out)
     return _window(list(out))
+
+
+def sector_window(sc, p: int, x: float, y: float, z: float) -> dict:
+    """PairGeneratorSector window of one inner hit (rows and phi cells after
+    the reference's unsigned / wrap-around corrections)."""
+    cfg = make_cfg(sc, 1)
+    out = (C.c_uint32 * 5)()
+    lib().oracle_sector_window(C.byref(cfg), p, x, y, z, out)
+    return {"zLowSector": out[0], "zHighSector": out[1], "phiLowSector": out[2],
+            "phiHighSector": out[3], "wrap": bool(out[4])}
 
 
 def predict_window(sc, lt: int, store: bool, p1, p2, min_r: float, max_r: float) -> dict:

@@ -267,6 +267,129 @@ static int pairs_stage(const oracle_cfg *c, oracle_result *r) {
   return 0;
 }
 
+// ------------------------------------------------------------ sector pairs
+// PairGeneratorSector.h:43-234, .cpp:5-151: every inner-layer hit with every
+// outer-layer hit in the grid cells of its (z row +- pairSpreadZ) x (phi
+// cell +- pairSpreadPhi) window, no geometric cut; n1*n2 bitmap per (event,
+// layer pair), per-work-item visit counts, scan, bitmap-ordered store.
+
+typedef struct {
+  uint32_t zLowSector, zHighSector, phiLowSector, phiHighSector;
+  int wrap;
+} sector_window_t;
+
+// pairCount window (PairGeneratorSector.h:100-115), with the OpenCL integer
+// types: zLowSector - spreadZ is unsigned and wraps below row 0 (the row loop
+// then runs zero times); the phi sector is an int from float arithmetic.
+static void sector_window(const oracle_cfg *c, uint32_t p, float X, float Y, float Z,
+                          sector_window_t *w) {
+  const uint32_t spreadZ = c->pair_spread_z[p], spreadPhi = c->pair_spread_phi[p];
+  const uint32_t nZ = c->n_z, nPhi = c->n_phi;
+  uint32_t zLowSector = o_f2u(floorf((Z - c->min_z) / c->sector_size_z));
+  uint32_t zHighSector = zLowSector + spreadZ + 1u;
+  zHighSector = zHighSector < nZ ? zHighSector : nZ;  // min(uint, uint)
+  zLowSector = zLowSector - spreadZ;                  // max(0u, .) of a uint
+  const float phi = o_atan2f(Y, X);
+  int32_t phiLowSector = o_f2i(floorf((phi - c->min_phi) / c->sector_size_phi) - (float)spreadPhi);
+  uint32_t phiHighSector = (uint32_t)phiLowSector + 2u * spreadPhi + 1u;
+  const int wrap = phiLowSector < 0 || phiHighSector > nPhi + 1u;
+  phiLowSector = (int32_t)((uint32_t)phiLowSector + (uint32_t)(phiLowSector < 0) * nPhi);
+  phiHighSector -= (uint32_t)(phiHighSector > nPhi + 1u) * nPhi;
+  w->zLowSector = zLowSector;
+  w->zHighSector = zHighSector;
+  w->phiLowSector = (uint32_t)phiLowSector;
+  w->phiHighSector = phiHighSector;
+  w->wrap = wrap;
+}
+
+static int sector_pairs_stage(const oracle_cfg *c, oracle_result *r) {
+  const uint32_t E = c->n_events, L = c->n_layers, P = c->n_triplets, T = c->threads;
+  const uint32_t nPhi = c->n_phi;
+  const uint64_t C = (uint64_t)(c->n_z + 1) * (nPhi + 1);
+  const uint64_t nCounts = (uint64_t)E * P * T + 1;
+  uint32_t *prefix = (uint32_t *)xcalloc(nCounts, 4);
+  r->pair_offsets = (uint32_t *)xcalloc((uint64_t)E * P + 1, 4);
+  if (!prefix || !r->pair_offsets) return -1;
+  uint32_t **bitmap = (uint32_t **)xcalloc((uint64_t)E * P, sizeof(uint32_t *));
+  if (!bitmap) return -1;
+  uint64_t raw_total = 0;
+  for (uint32_t e = 0; e < E; ++e)
+    for (uint32_t p = 0; p < P; ++p) {
+      const uint64_t k = (uint64_t)e * P + p;
+      const uint32_t *g1 = r->grid + ((uint64_t)e * L + (c->layer1[p] - 1)) * C;
+      const uint32_t *g2 = r->grid + ((uint64_t)e * L + (c->layer2[p] - 1)) * C;
+      // inner hits up to the last cell's start; outer likewise (:67-81)
+      const uint32_t layer1Offset = g1[0], end = g1[C - 1];
+      const uint32_t offset = g2[0], nHits2 = g2[C - 1] - g2[0];
+      const uint64_t nbits = (uint64_t)(end - layer1Offset) * nHits2;
+      bitmap[k] = (uint32_t *)xcalloc((nbits + 31) / 32 + 1, 4);
+      if (!bitmap[k]) return -1;
+      // work-item t handles inner hits layer1Offset + t, + T, ... (:95-146)
+      for (uint32_t i = layer1Offset; i < end; ++i) {
+        const uint32_t t = (i - layer1Offset) % T;
+        sector_window_t w;
+        sector_window(c, p, r->x[i], r->y[i], r->z[i], &w);
+        uint32_t nFound = 0;
+        for (uint32_t zSector = w.zLowSector; zSector < w.zHighSector; ++zSector) {
+          const uint32_t row = zSector * (nPhi + 1);
+          const uint32_t zSectorStart = g2[row];
+          const uint32_t zSectorEnd = g2[row + nPhi];  // (zSector+1)(nPhi+1)-1
+          const uint32_t zSectorLength = zSectorEnd - zSectorStart;
+          uint32_t j = g2[row + w.phiLowSector];
+          const uint32_t end2 = (uint32_t)w.wrap * zSectorEnd + g2[row + w.phiHighSector] -
+                                (uint32_t)w.wrap * zSectorStart;
+          for (; j < end2; ++j) {
+            ++nFound;
+            const uint32_t index =
+                (i - layer1Offset) * nHits2 + j - (uint32_t)(j >= zSectorEnd) * zSectorLength - offset;
+            bitmap[k][index / 32] |= 1u << (index % 32);
+          }
+        }
+        prefix[k * T + t] += nFound;
+        raw_total += nFound;
+      }
+    }
+  oracle_scan_u32(prefix, nCounts);
+  const uint32_t nPairs = prefix[nCounts - 1];
+  r->n_pairs = nPairs;
+  r->pairs = (uint32_t *)xcalloc(2 * (uint64_t)nPairs + 2, 4);
+  if (!r->pairs) return -1;
+  memset(r->pairs, 0xFF, 8 * (uint64_t)nPairs);
+  uint64_t written = 0;
+  // pairStore (PairGeneratorSector.h:157-234): per work item, every inner hit
+  // of its stride tests all n2 bits and writes (inner, outer) while pos < next
+  for (uint32_t e = 0; e < E; ++e)
+    for (uint32_t p = 0; p < P; ++p) {
+      const uint64_t k = (uint64_t)e * P + p;
+      const uint32_t *g1 = r->grid + ((uint64_t)e * L + (c->layer1[p] - 1)) * C;
+      const uint32_t *g2 = r->grid + ((uint64_t)e * L + (c->layer2[p] - 1)) * C;
+      const uint32_t layer1Offset = g1[0], end = g1[C - 1];
+      const uint32_t offset = g2[0], nHits2 = g2[C - 1] - g2[0];
+      for (uint32_t t = 0; t < T; ++t) {
+        uint32_t pos = prefix[k * T + t];
+        const uint32_t next = prefix[k * T + t + 1];
+        for (uint32_t i = layer1Offset + t; i < end; i += T)
+          for (uint32_t j = 0; j < nHits2 && pos < next; ++j) {
+            const uint32_t index = (i - layer1Offset) * nHits2 + j;
+            const int valid = (bitmap[k][index / 32] >> (index % 32)) & 1u;
+            if (valid) {
+              r->pairs[2 * (uint64_t)pos] = i;
+              r->pairs[2 * (uint64_t)pos + 1] = offset + j;
+              ++written;
+            }
+            pos += (uint32_t)valid;
+          }
+      }
+      r->pair_offsets[k + 1] = prefix[k * T + T];
+    }
+  r->n_pairs_unwritten = (uint32_t)(nPairs - written);
+  r->n_pairs_raw_overcount = (uint32_t)(raw_total - written);
+  for (uint64_t k = 0; k < (uint64_t)E * P; ++k) free(bitmap[k]);
+  free(bitmap);
+  free(prefix);
+  return 0;
+}
+
 // ---------------------------------------------------------------- prediction
 typedef struct {
   float zLow, zHigh, phiLow, phiHigh;
@@ -539,13 +662,18 @@ int oracle_run_chain(const oracle_cfg *c, const oracle_input *in, int stop_after
   if (c->n_layers == 0 || c->n_layers > ORACLE_MAX_LAYERS || c->n_triplets > ORACLE_MAX_TRIPLETS ||
       c->threads == 0)
     return -2;
+  // the sector window indexes inside its grid row only for 2 * spread <= nPhi
+  if (c->pair_generator == ORACLE_PAIRS_SECTOR)
+    for (uint32_t p = 0; p < c->n_triplets; ++p)
+      if (2u * c->pair_spread_phi[p] > c->n_phi) return -2;
   for (uint32_t p = 0; p < c->n_triplets; ++p)
     if (c->layer1[p] < 1 || c->layer1[p] > c->n_layers || c->layer2[p] < 1 ||
         c->layer2[p] > c->n_layers || c->layer3[p] < 1 || c->layer3[p] > c->n_layers)
       return -2;
   if (grid_build(c, in, out)) return -1;
   if (stop_after < 1) return 0;
-  if (pairs_stage(c, out)) return -1;
+  if ((c->pair_generator == ORACLE_PAIRS_SECTOR ? sector_pairs_stage(c, out) : pairs_stage(c, out)))
+    return -1;
   if (stop_after < 2) return 0;
   if (predict_stage(c, out)) return -1;
   if (stop_after < 3) return 0;
@@ -586,6 +714,16 @@ void oracle_eval_f32(int op, const float *a, const float *b, float *out, uint64_
       default: out[i] = 0.0f;
     }
   }
+}
+
+void oracle_sector_window(const oracle_cfg *c, uint32_t p, float x, float y, float z, uint32_t *out) {
+  sector_window_t w;
+  sector_window(c, p, x, y, z, &w);
+  out[0] = w.zLowSector;
+  out[1] = w.zHighSector;
+  out[2] = w.phiLowSector;
+  out[3] = w.phiHighSector;
+  out[4] = (uint32_t)w.wrap;
 }
 
 void oracle_pair_window(const oracle_cfg *c, uint32_t p, float x, float y, float z, float *out) {

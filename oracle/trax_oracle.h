@@ -10,7 +10,8 @@
 //   grid build      algorithms/GridBuilder.h:48-257, GridBuilder.cpp:6-160
 //   prefix sum      algorithms/PrefixSum.h:55-245, PrefixSum.cpp:8-126
 //   pairs           algorithms/PairGeneratorBeamspot.h:43-246 (count),
-//                   :448-525 (store), PairGeneratorBeamspot.cpp:5-177
+//                   :448-525 (store), PairGeneratorBeamspot.cpp:5-177;
+//                   or algorithms/PairGeneratorSector.h:43-234, .cpp:5-151
 //   prediction      algorithms/TripletThetaPhiPredictor.h:46-248 (count),
 //                   :250-481 (store), :483-497 (offsets), .cpp:10-131
 //   filter          algorithms/TripletThetaPhiFilter.h:50-313, .cpp:10-164
@@ -38,6 +39,8 @@ extern "C" {
 #define ORACLE_MAX_LAYERS 16
 #define ORACLE_MAX_TRIPLETS 8
 #define ORACLE_UNWRITTEN 0xFFFFFFFFu
+#define ORACLE_PAIRS_BEAMSPOT 0u  // PairGeneratorBeamspot
+#define ORACLE_PAIRS_SECTOR 1u    // PairGeneratorSector
 
 typedef struct {
   uint32_t n_events;
@@ -54,6 +57,7 @@ typedef struct {
   uint32_t pair_spread_z[ORACLE_MAX_TRIPLETS], pair_spread_phi[ORACLE_MAX_TRIPLETS];
   float min_radius_curvature;
   uint32_t threads;  // reference work-group size (orders pairs; no effect on sets)
+  uint32_t pair_generator;  // ORACLE_PAIRS_BEAMSPOT or ORACLE_PAIRS_SECTOR
 } oracle_cfg;
 
 typedef struct {
@@ -102,6 +106,9 @@ uint32_t oracle_cell_of(const oracle_cfg *cfg, float x, float y, float z, float 
 // selects the store-side variant).  out: zLow, zHigh, phiLow, phiHigh, wrap,
 // zLowSector, zHighSector, phiLowSector, phiHighSector.
 void oracle_pair_window(const oracle_cfg *cfg, uint32_t p, float x, float y, float z, float *out);
+// Sector window of one inner hit: out = zLowSector, zHighSector, phiLowSector,
+// phiHighSector, wrap (after the reference's corrections).
+void oracle_sector_window(const oracle_cfg *cfg, uint32_t p, float x, float y, float z, uint32_t *out);
 void oracle_predict_window(const oracle_cfg *cfg, uint32_t lt, int store, const float *p1,
                            const float *p2, float minR, float maxR, float *out);
 // Filter quantities of one triplet: out = theta ratio, |dphi|, tip^2, circle radius, pass.

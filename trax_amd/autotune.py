@@ -84,7 +84,7 @@ def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps:
           "pairs_two_phase")
     sweep("predict_block", BLOCKS,
           lambda: (rt.predict_count(ctx, bufs, E), rt.predict_store(ctx, bufs, E)), "predict_two_phase")
-    stage_ms["pairs"] = _time(lambda: rt.pairs_beamspot(ctx, bufs, E), reps, torch)
+    stage_ms["pairs"] = _time(lambda: rt.pair_stage(ctx, bufs, E), reps, torch)
     stage_ms["predict"] = _time(lambda: rt.triplet_predict(ctx, bufs, E), reps, torch)
     ctx.set_launch(**best)
     stage_ms["filter"] = _time(lambda: rt.triplet_filter(ctx, bufs, E), reps, torch)

@@ -20,7 +20,7 @@ LIB = os.path.join(BUILD, "libtraxhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TRAX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["scan.hip", "grid.hip", "pairs.hip", "predict.hip", "filter.hip", "capi.hip"]
+SOURCES = ["scan.hip", "grid.hip", "pairs.hip", "sector.hip", "predict.hip", "filter.hip", "capi.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

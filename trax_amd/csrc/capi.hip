@@ -152,7 +152,15 @@ int make_params(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events, t
     // the pair kernel's sigmaPhi argument is bound to the uint pairSpreadPhi
     // column (PairGeneratorBeamspot.cpp:55): its bits are read as a float
     memcpy(&sp->pair_sigma_phi[p], &t.pair_spread_phi, 4);
+    sp->spread_z[p] = t.pair_spread_z;
+    sp->spread_phi[p] = t.pair_spread_phi;
+    // the sector window stays inside its grid row only for 2 * spread <= nPhi
+    // (the reference indexes neighbouring rows beyond that)
+    if (cfg->pair_generator == TRAX_PAIRS_SECTOR && 2ull * t.pair_spread_phi > cfg->n_phi)
+      return fail(ctx, TRAX_ERR_INVALID, "sector pairs need 2 * pair_spread_phi <= n_phi");
   }
+  if (cfg->pair_generator != TRAX_PAIRS_BEAMSPOT && cfg->pair_generator != TRAX_PAIRS_SECTOR)
+    return fail(ctx, TRAX_ERR_INVALID, "unknown pair generator");
   return TRAX_OK;
 }
 
@@ -490,6 +498,58 @@ extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, u
   return TRAX_OK;
 }
 
+// ------------------------------------------------------------- sector pairs
+extern "C" int trax_pairs_sector_count(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                       const trax_hits* sorted, const uint32_t* d_grid,
+                                       const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals)
+    return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t K = n_events * cfg->n_triplets;
+  TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s, true));
+  TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_sector_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
+                                                ws->pair_counts, nullptr, 0, ws->max_hits, false,
+                                                ctx->flags, s));
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
+                                            ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s,
+                                            scan_sub(ctx)));
+  // pair_offsets[k] = scanned count at the first inner hit of segment k
+  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->pair_counts, ws->max_hits, ws->work_offsets + K,
+                                                  ws->work_offsets, K + 1, out->pair_offsets,
+                                                  out->totals + 0, s));
+  return TRAX_OK;
+}
+
+extern "C" int trax_pairs_sector_store(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                       const trax_hits* sorted, const uint32_t* d_grid,
+                                       const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if (!sorted_ok(sorted) || !d_grid || !out || (!out->pairs && out->pair_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
+  TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, static_cast<hipStream_t>(stream),
+             trax::launch_sector_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets, ws->pair_counts,
+                                       reinterpret_cast<uint2*>(out->pairs), out->pair_capacity,
+                                       ws->max_hits, true, ctx->flags, static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
+extern "C" int trax_pairs_sector(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                 const trax_hits* sorted, const uint32_t* d_grid,
+                                 const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = trax_pairs_sector_count(ctx, cfg, n_events, sorted, d_grid, ws, out, stream);
+  if (rc) return rc;
+  return trax_pairs_sector_store(ctx, cfg, n_events, sorted, d_grid, ws, out, stream);
+}
+
 // ---------------------------------------------------------------- predictor
 extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
                                     const trax_hits* sorted, const uint32_t* d_grid,
@@ -646,7 +706,10 @@ extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const
   int rc = trax_grid_build(ctx, cfg, in, layout, sorted, d_grid, stream);
   if (rc) return rc;
   const uint32_t E = layout->n_events;
-  if ((rc = trax_pairs_beamspot(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
+  rc = cfg->pair_generator == TRAX_PAIRS_SECTOR
+           ? trax_pairs_sector(ctx, cfg, E, sorted, d_grid, ws, out, stream)
+           : trax_pairs_beamspot(ctx, cfg, E, sorted, d_grid, ws, out, stream);
+  if (rc) return rc;
   if ((rc = trax_triplet_predict(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
   return trax_triplet_filter(ctx, cfg, E, sorted, ws, out, stream);
 }

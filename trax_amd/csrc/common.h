@@ -30,6 +30,7 @@ struct SeedParams {
   float dphi_cut[TRAX_MAX_TRIPLETS], sigma_phi[TRAX_MAX_TRIPLETS];
   float tip_cut[TRAX_MAX_TRIPLETS], z0[TRAX_MAX_TRIPLETS];
   float pair_sigma_phi[TRAX_MAX_TRIPLETS];  // pairSpreadPhi bits read as float
+  uint32_t spread_z[TRAX_MAX_TRIPLETS], spread_phi[TRAX_MAX_TRIPLETS];  // sector pairs
   // acos(radius / (2 minRadiusCurvature)) per layer for the min and max
   // radius: window terms that depend on the layer only, evaluated once on
   // the host with the same FP32 definition the kernels use

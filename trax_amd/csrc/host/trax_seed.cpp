@@ -52,8 +52,8 @@ void put_u32(FILE* f, uint32_t v) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    fprintf(stderr, "usage: %s <events.bin> <results.bin>\n", argv[0]);
+  if (argc != 3 && !(argc == 4 && (std::string(argv[3]) == "beamspot" || std::string(argv[3]) == "sector"))) {
+    fprintf(stderr, "usage: %s <events.bin> <results.bin> [beamspot|sector]\n", argv[0]);
     return 2;
   }
   try {
@@ -109,8 +109,14 @@ int main(int argc, char** argv) {
 
     trax::GridBuilder gridBuilder(ctx, ws);
     gridBuilder.run(hits, 256, es, ls, grid, geom, trip);
-    trax::PairGeneratorBeamspot pairGen(ctx, ws, tot);
-    auto pairs = pairGen.run(hits, geom, 256, trip, grid);
+    std::unique_ptr<trax::Pairing> pairs;
+    if (argc == 4 && std::string(argv[3]) == "sector") {
+      trax::PairGeneratorSector pairGen(ctx, ws, tot);
+      pairs = pairGen.run(hits, geom, 256, trip, grid);
+    } else {
+      trax::PairGeneratorBeamspot pairGen(ctx, ws, tot);
+      pairs = pairGen.run(hits, geom, 256, trip, grid);
+    }
     trax::TripletThetaPhiPredictor predictor(ctx, ws, tot);
     auto cands = predictor.run(hits, geom, 256, trip, grid, *pairs);
     trax::TripletThetaPhiFilter filter(ctx, ws, tot);

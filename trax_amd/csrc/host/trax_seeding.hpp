@@ -4,6 +4,7 @@
 // (paths relative to the reference's src/):
 //   GridBuilder::run                  algorithms/GridBuilder.h:42
 //   PairGeneratorBeamspot::run        algorithms/PairGeneratorBeamspot.h:40-41
+//   PairGeneratorSector::run          algorithms/PairGeneratorSector.h:38-39
 //   TripletThetaPhiPredictor::run     algorithms/TripletThetaPhiPredictor.h:43-44
 //   TripletThetaPhiFilter::run        algorithms/TripletThetaPhiFilter.h:46-48
 //   PrefixSum::run                    algorithms/PrefixSum.h:42
@@ -344,6 +345,40 @@ class PairGeneratorBeamspot : public KernelWrapper {
     o.pair_offsets = pairs->pairingOffsets.get();
     ctx.check(trax_pairs_store(ctx.ctx(), &cfg, E, &s, grid.boundary.get(), ws.get(), &o, ctx.stream()),
               "pairs store");
+    return pairs;
+  }
+
+ private:
+  Workspace& ws;
+  Totals& tot;
+};
+
+class PairGeneratorSector : public KernelWrapper {
+ public:
+  PairGeneratorSector(Context& c, Workspace& w, Totals& t) : KernelWrapper(c), ws(w), tot(t) {}
+  // PairGeneratorSector.h:38-39 (same arguments as the beam-spot generator):
+  // count, read the total, allocate, store.
+  std::unique_ptr<Pairing> run(HitCollection& hits, const GeometrySupplement& geom, uint32_t /*nThreads*/,
+                               const TripletConfigurations& t, const Grid& grid) {
+    trax_seed_config cfg = make_config(grid.config, geom, t);
+    cfg.pair_generator = TRAX_PAIRS_SECTOR;
+    const uint32_t E = grid.config.nEvents, P = (uint32_t)t.size();
+    ws.ensure(cfg, E, (uint32_t)hits.size(), 1);
+    trax_hits s = hits.sorted();
+    auto offs = std::make_unique<Pairing>(0, E, P);
+    trax_seed_outputs o{};
+    o.pair_offsets = offs->pairingOffsets.get();
+    o.totals = tot.d.get();
+    ctx.check(trax_pairs_sector_count(ctx.ctx(), &cfg, E, &s, grid.boundary.get(), ws.get(), &o, ctx.stream()),
+              "sector pairs count");
+    const uint32_t n = tot.read(ctx, 0);
+    auto pairs = std::make_unique<Pairing>(n, E, P);
+    std::swap(pairs->pairingOffsets, offs->pairingOffsets);
+    o.pairs = pairs->pairing.get();
+    o.pair_capacity = n;
+    o.pair_offsets = pairs->pairingOffsets.get();
+    ctx.check(trax_pairs_sector_store(ctx.ctx(), &cfg, E, &s, grid.boundary.get(), ws.get(), &o, ctx.stream()),
+              "sector pairs store");
     return pairs;
   }
 

@@ -42,7 +42,12 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
 hipError_t grid_build_set_lds_limit(size_t bytes);
 
 hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t* work_off,
-                            hipStream_t stream);
+                            hipStream_t stream, bool inner = false);
+// Sector pairs (count or store pass over the inner-hit work list).
+hipError_t launch_sector_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                               const uint32_t* work_off, uint32_t* counts, uint2* pairs,
+                               uint64_t capacity, uint32_t max_work, bool store, uint32_t* flags,
+                               hipStream_t stream);
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                         const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
                         uint32_t max_work, bool store, uint32_t* flags, uint32_t block,

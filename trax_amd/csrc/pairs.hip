@@ -86,10 +86,11 @@ __device__ __forceinline__ bool window_phi_ok(bool wrap, float phiLow, float phi
          (wrap && ((phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= phiHigh)));
 }
 
-// Per-(event, layer pair) work offsets: prefix sum of outer-layer hit counts.
-// nHits2 = grid[C-1] - grid[0] of the outer layer (PairGeneratorBeamspot.h:70-73).
+// Per-(event, layer pair) work offsets: prefix sum of the outer-layer hit
+// counts grid[C-1] - grid[0] (PairGeneratorBeamspot.h:70-73), or with `inner`
+// of the inner layer's (PairGeneratorSector.h:67-70).
 __global__ __launch_bounds__(256) void k_pair_plan(SeedParams sp, const uint32_t* __restrict__ grid,
-                                                   uint32_t* __restrict__ work_off) {
+                                                   uint32_t* __restrict__ work_off, bool inner) {
   __shared__ uint32_t s_wave[4];
   const uint32_t K = sp.n_events * sp.n_triplets, C = sp.cells, L = sp.n_layers;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
@@ -99,8 +100,9 @@ __global__ __launch_bounds__(256) void k_pair_plan(SeedParams sp, const uint32_t
     uint32_t v = 0;
     if (k < K) {
       const uint32_t e = k / sp.n_triplets, p = k % sp.n_triplets;
-      const uint32_t* g2 = grid + ((uint64_t)e * L + (sp.layer2[p] - 1)) * C;
-      v = g2[C - 1] - g2[0];
+      const uint32_t layer = inner ? sp.layer1[p] : sp.layer2[p];
+      const uint32_t* g = grid + ((uint64_t)e * L + (layer - 1)) * C;
+      v = g[C - 1] - g[0];
     }
     uint32_t incl = wave_inclusive_scan(v);
     if (lane == 63) s_wave[wave] = incl;
@@ -293,8 +295,8 @@ hipError_t launch_read_last(const uint32_t* arr, const uint32_t* d_index, uint32
 }
 
 hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t* work_off,
-                            hipStream_t stream) {
-  hipLaunchKernelGGL(k_pair_plan, dim3(1), dim3(256), 0, stream, sp, grid, work_off);
+                            hipStream_t stream, bool inner) {
+  hipLaunchKernelGGL(k_pair_plan, dim3(1), dim3(256), 0, stream, sp, grid, work_off, inner);
   return hipGetLastError();
 }
 

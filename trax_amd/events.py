@@ -73,6 +73,9 @@ class SeedConfig:
     triplets: List[LayerTriplet]
     min_pt: float = 1.0
     threads: int = 256  # work-group size of the reference (orders pairs only)
+    # pair generator: "beamspot" (PairGeneratorBeamspot) or "sector"
+    # (PairGeneratorSector, pairSpreadZ x pairSpreadPhi cells, no cut)
+    pair_generator: str = "beamspot"
 
     @property
     def n_layers(self) -> int:

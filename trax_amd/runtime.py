@@ -49,7 +49,10 @@ class SeedConfigC(C.Structure):
     _fields_ = [("n_z", C.c_uint32), ("n_phi", C.c_uint32), ("n_layers", C.c_uint32),
                 ("min_radius", C.c_float * MAX_LAYERS), ("max_radius", C.c_float * MAX_LAYERS),
                 ("n_triplets", C.c_uint32), ("triplets", LayerTripletC * MAX_TRIPLETS),
-                ("min_radius_curvature", C.c_float)]
+                ("min_radius_curvature", C.c_float), ("pair_generator", C.c_uint32)]
+
+
+PAIR_GENERATORS = {"beamspot": 0, "sector": 1}  # TRAX_PAIRS_BEAMSPOT / TRAX_PAIRS_SECTOR
 
 
 class HitsC(C.Structure):
@@ -106,6 +109,12 @@ SIGNATURES = {
                                    _P(OutputsC), _vp]),
     "trax_pairs_beamspot": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
                                       _P(WorkspaceC), _P(OutputsC), _vp]),
+    "trax_pairs_sector_count": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
+                                          _P(WorkspaceC), _P(OutputsC), _vp]),
+    "trax_pairs_sector_store": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
+                                          _P(WorkspaceC), _P(OutputsC), _vp]),
+    "trax_pairs_sector": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
+                                    _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_triplet_predict": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
                                        _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_predict_count": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
@@ -161,6 +170,7 @@ def seed_config_c(sc: ev.SeedConfig) -> SeedConfigC:
         r.pair_spread_phi = t.pair_spread_phi
         r.n_candidates = t.n_candidates
     c.min_radius_curvature = float(sc.min_radius_curvature)
+    c.pair_generator = PAIR_GENERATORS[sc.pair_generator]
     return c
 
 
@@ -457,11 +467,21 @@ def _stage(ctx, fn, name, bufs, n_events, with_grid=True, stream=None):
 
 
 def pairs_count(ctx, bufs, n_events, stream=None):
-    _stage(ctx, ctx.lib.trax_pairs_count, "trax_pairs_count", bufs, n_events, stream=stream)
+    """Count phase of the configured pair generator (beam-spot or sector)."""
+    if bufs.sc.pair_generator == "sector":
+        _stage(ctx, ctx.lib.trax_pairs_sector_count, "trax_pairs_sector_count", bufs, n_events,
+               stream=stream)
+    else:
+        _stage(ctx, ctx.lib.trax_pairs_count, "trax_pairs_count", bufs, n_events, stream=stream)
 
 
 def pairs_store(ctx, bufs, n_events, stream=None):
-    _stage(ctx, ctx.lib.trax_pairs_store, "trax_pairs_store", bufs, n_events, stream=stream)
+    """Store phase of the configured pair generator."""
+    if bufs.sc.pair_generator == "sector":
+        _stage(ctx, ctx.lib.trax_pairs_sector_store, "trax_pairs_sector_store", bufs, n_events,
+               stream=stream)
+    else:
+        _stage(ctx, ctx.lib.trax_pairs_store, "trax_pairs_store", bufs, n_events, stream=stream)
 
 
 def predict_count(ctx, bufs, n_events, stream=None):
@@ -475,6 +495,20 @@ def predict_store(ctx, bufs, n_events, stream=None):
 def pairs_beamspot(ctx, bufs, n_events, stream=None):
     """Single-pass pair stage (count, positions and store in one kernel)."""
     _stage(ctx, ctx.lib.trax_pairs_beamspot, "trax_pairs_beamspot", bufs, n_events, stream=stream)
+
+
+def pairs_sector(ctx, bufs, n_events, stream=None):
+    """PairGeneratorSector (PairGeneratorSector.h:38-39): count, scan and store
+    with no host read."""
+    _stage(ctx, ctx.lib.trax_pairs_sector, "trax_pairs_sector", bufs, n_events, stream=stream)
+
+
+def pair_stage(ctx, bufs, n_events, stream=None):
+    """Single-call pair stage of the configured generator."""
+    if bufs.sc.pair_generator == "sector":
+        pairs_sector(ctx, bufs, n_events, stream)
+    else:
+        pairs_beamspot(ctx, bufs, n_events, stream)
 
 
 def triplet_predict(ctx, bufs, n_events, stream=None):
