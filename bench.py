@@ -45,6 +45,8 @@ def parse():
                     help="short run for rocprofv3: no plain path, no CPU baseline, a 256 MiB "
                          "copy kernel for counter calibration")
     ap.add_argument("--tune", default=None, help="autotuning JSON to apply (trax_amd.autotune)")
+    ap.add_argument("--pair-generator", choices=["beamspot", "sector"], default="beamspot",
+                    help="pair stage: PairGeneratorBeamspot (default) or PairGeneratorSector")
     return ap.parse_args()
 
 
@@ -70,6 +72,9 @@ ALGO_BYTES = {
     "grid_build": lambda c: 64 * c["hits"] + 4 * c["cells"],
     # x, y, z, phi of every hit and the cells read, one uint2 per pair written
     "pairs": lambda c: 16 * c["hits"] + 4 * c["cells"] + 8 * c["pairs"],
+    # sector pairs, count + store passes: z and phi of every hit read twice,
+    # its count written and read, the cells read, one uint2 per pair written
+    "pairs_sector": lambda c: 24 * c["hits"] + 4 * c["cells"] + 8 * c["pairs"],
     # pair read + count written per pair, z and phi of every hit and the cells
     # read, one stash slot per candidate slot written
     "predict_walk": lambda c: 16 * c["hits"] + 4 * c["cells"] + 12 * c["pairs"] + 4 * c["cands"],
@@ -142,6 +147,9 @@ def main():
         args.no_plain = args.no_cpu = True
     w = ev.config_workload(args.config, args.events)
     sc = w.seed_config
+    sc.pair_generator = args.pair_generator
+    if args.pair_generator != "beamspot":
+        w.name += "-" + args.pair_generator
     log(f"generating {w.n_events} events ({w.name})")
     batch = ev.make_batch(w)
     log(f"{batch.n_hits} hits")
@@ -218,8 +226,9 @@ def main():
                  "filter_store", "scan"):
         ms_launch, n = ctx.time_kernel(name, lambda: runtime.seed_chain(ctx, bufs, db), reps)
         kern[name] = {"ms": ms_launch, "launches_per_step": n / reps}
-        if name in ALGO_BYTES:
-            b = ALGO_BYTES[name](counts)
+        key = "pairs_sector" if name == "pairs" and sc.pair_generator == "sector" else name
+        if key in ALGO_BYTES:
+            b = ALGO_BYTES[key](counts) / max(1.0, n / reps)
             kern[name].update(bytes=b, gbs=b / (ms_launch * 1e-3) / 1e9)
     torch.cuda.synchronize()
     if ctx.flags():
