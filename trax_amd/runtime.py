@@ -422,6 +422,21 @@ def fetch(bufs: ChainBuffers, n_events: int, n_hits: int) -> Dict[str, np.ndarra
     }
 
 
+def fetch_triplets(bufs: ChainBuffers, n_events: int, n_hits: int) -> Dict[str, np.ndarray]:
+    """Copy only the chain's result to the host: the triplets (as sorted-hit
+    indexes), their per-(event, layer triplet) offsets and the sorted-to-input
+    index map."""
+    n_t = totals(bufs)[2]
+    K1 = n_events * len(bufs.sc.triplets) + 1
+
+    def u(t, n):
+        return t[:n].cpu().numpy().view(np.uint32).copy()
+
+    return {"orig_index": u(bufs.sorted["orig_index"], n_hits),
+            "tracklets": np.stack([u(bufs.t[i], n_t) for i in range(3)], axis=1),
+            "tracklet_offsets": u(bufs.tracklet_offsets, K1)}
+
+
 def prefix_sum(ctx: Context, t, stream=None) -> None:
     """In-place exclusive scan of a 16-byte aligned int32/uint32 device tensor."""
     ctx.check(ctx.lib.trax_prefix_sum(ctx.h, _vp(t.data_ptr()), t.numel(), stream_ptr(stream)),

@@ -53,7 +53,8 @@ class Slot:
         self.requests: List[Request] = []
         self.done_event = None
         self.db: Optional[rt.DeviceBatch] = None
-        self.batch: Optional[ev.EventBatch] = None
+        self.offsets = np.zeros(0, np.int64)  # first hit of each request in the batch
+        self.n_hits = 0
 
     @property
     def busy(self) -> bool:
@@ -67,22 +68,37 @@ class Slot:
         return int(tot)
 
     def launch(self, reqs: List[Request]) -> None:
+        """Gather the requests' hits straight into this slot's pinned staging
+        buffers (one copy per event and column, event index renumbered),
+        upload them and enqueue the chain on the slot's stream."""
         torch = self.torch
-        batch = ev.concat_events([r.batch for r in reqs])
-        n = batch.n_hits
-        for k, col in zip(("x", "y", "z", "layer", "det_id", "hit_id", "evt"), batch.columns()):
-            h = self.host[k]
-            src = torch.from_numpy(col.view(np.int32) if col.dtype == np.uint32 else col)
-            h[:n].copy_(src)
-        E = batch.n_events
-        for k, a in (("event_offset", batch.event_offset), ("layer_nhits", batch.layer_nhits),
-                     ("layer_offset", batch.layer_offset)):
-            self.host_layout[k][:a.shape[0]].copy_(torch.from_numpy(a.view(np.int32)))
+        L = self.bufs.sc.n_layers
+        hv = {k: t.numpy().view(np.uint32) if k not in "xyz" else t.numpy() for k, t in self.host.items()}
+        lay = {k: t.numpy().view(np.uint32) for k, t in self.host_layout.items()}
+        n = 0
+        offsets = np.zeros(len(reqs), np.int64)
+        for k, r in enumerate(reqs):
+            b = r.batch
+            m = b.n_hits
+            sl = slice(n, n + m)
+            hv["x"][sl] = b.x
+            hv["y"][sl] = b.y
+            hv["z"][sl] = b.z
+            hv["layer"][sl] = b.layer
+            hv["det_id"][sl] = b.det_id
+            hv["hit_id"][sl] = b.hit_id
+            hv["evt"][sl] = k
+            lay["event_offset"][k] = n
+            lay["layer_nhits"][k * L:(k + 1) * L] = b.layer_nhits
+            lay["layer_offset"][k * L:(k + 1) * L] = b.layer_offset
+            offsets[k] = n
+            n += m
+        E = len(reqs)
         with torch.cuda.stream(self.stream):
             for k in self.dev:
                 self.dev[k][:n].copy_(self.host[k][:n], non_blocking=True)
             for k in self.dev_layout:
-                m = E if k == "event_offset" else E * batch.n_layers
+                m = E if k == "event_offset" else E * L
                 self.dev_layout[k][:m].copy_(self.host_layout[k][:m], non_blocking=True)
             cols = dict(self.dev)
             cols.update(self.dev_layout)
@@ -91,14 +107,15 @@ class Slot:
             self.done_event = torch.cuda.Event()
             self.done_event.record(self.stream)
         self.requests = reqs
-        self.batch = batch
+        self.offsets = offsets
+        self.n_hits = n
 
     def ready(self) -> bool:
         return self.busy and self.done_event.query()
 
     def finish(self, now: float) -> List[Request]:
         """Collect the results of the finished batch (its stream is idle)."""
-        reqs, batch = self.requests, self.batch
+        reqs = self.requests
         flags = self.ctx.flags(stream=self.stream)
         if flags & rt.FLAG_LAYOUT:
             raise rt.TraxError("inconsistent event layout")
@@ -115,15 +132,13 @@ class Slot:
                 raise rt.TraxError("slot still overflows after regrowing")
         P = len(self.bufs.sc.triplets)
         with self.torch.cuda.stream(self.stream):
-            res = rt.fetch(self.bufs, batch.n_events, batch.n_hits)
+            res = rt.fetch_triplets(self.bufs, len(reqs), self.n_hits)
         offs = res["tracklet_offsets"]
+        trk = res["orig_index"][res["tracklets"]].astype(np.int64)
         for e, r in enumerate(reqs):
-            lo, hi = offs[e * P], offs[(e + 1) * P]
-            base = int(batch.event_offset[e])
-            r.triplets = res["orig_index"][res["tracklets"][lo:hi]].astype(np.int64) - base
+            r.triplets = trk[offs[e * P]:offs[(e + 1) * P]] - self.offsets[e]
             r.done = now
         self.requests = []
-        self.batch = None
         return reqs
 
 
@@ -210,3 +225,56 @@ def poisson_requests(w: ev.Workload, n: int, rate_hz: float, seed: int = ev.SEED
     rng = np.random.default_rng([seed, 0xA77, w.config_id])
     t = np.cumsum(rng.exponential(1.0 / rate_hz, n))
     return [Request(k, float(t[k]), ev.make_batch(w, events=[k], seed=seed)) for k in range(n)]
+
+
+def _check_against_plain(ctx, sc: ev.SeedConfig, reqs: Sequence[Request]) -> int:
+    """Every request's served triplets equal the plain method's (one event,
+    stage by stage with host reads) on the same event; returns the count."""
+    for r in reqs:
+        bufs, db = rt.run_batch(ctx, sc, r.batch)
+        rt.run_plain_event(ctx, bufs, db)
+        alone = rt.fetch_triplets(bufs, 1, r.batch.n_hits)
+        t = alone["orig_index"][alone["tracklets"]].astype(np.int64)
+        key = lambda a: a[np.lexsort((a[:, 2], a[:, 1], a[:, 0]))] if len(a) else a.reshape(0, 3)
+        if not np.array_equal(key(r.triplets.reshape(-1, 3)), key(t.reshape(-1, 3))):
+            raise rt.TraxError(f"request {r.rid}: served triplets differ from the plain path")
+    return len(reqs)
+
+
+def main(argv=None) -> int:
+    """Serving benchmark: a Poisson stream of the PU20/PU200 mix at several
+    arrival rates (and all requests at t = 0), one line of JSON per rate."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(prog="python -m trax_amd.serving")
+    ap.add_argument("--events", type=int, default=4096)
+    ap.add_argument("--rates", default="5000,20000,50000,0",
+                    help="arrival rates in events/s; 0 = every request at t = 0")
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--max-batch", type=int, default=512)
+    ap.add_argument("--slots", type=int, default=3)
+    ap.add_argument("--slot-hits", type=int, default=2_000_000)
+    ap.add_argument("--check", type=int, default=64, help="served events checked against the plain path")
+    args = ap.parse_args(argv)
+    import torch
+    w = ev.config_workload(4, args.events)
+    srv = SeedServer(w.seed_config, budget_bytes=int(args.budget_gb * (1 << 30)), max_batch=args.max_batch,
+                     n_slots=args.slots, slot_hits=args.slot_hits)
+    ctx = rt.Context(torch.cuda.current_device())
+    for rate in [float(x) for x in args.rates.split(",")]:
+        srv.serve(poisson_requests(w, 64, 1e9, seed=ev.SEED_BASE + 1), realtime=False)  # warm-up
+        reqs = poisson_requests(w, args.events, rate if rate > 0 else 1e9)
+        st = srv.serve(reqs, realtime=rate > 0)
+        checked = _check_against_plain(ctx, w.seed_config, reqs[:: max(1, len(reqs) // args.check)])
+        print(json.dumps({"workload": w.name, "arrival_rate_hz": rate or None, "events": st.n_events,
+                          "events_per_sec": st.events_per_sec, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
+                          "batches": st.batches, "mean_batch": st.mean_batch, "max_batch": st.max_batch,
+                          "slots": st.slots, "budget_gb": st.budget_bytes / (1 << 30),
+                          "used_gb": st.used_bytes / (1 << 30), "checked_vs_plain": checked}), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
