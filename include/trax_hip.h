@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TRAX_ABI_VERSION 3
+#define TRAX_ABI_VERSION 4
 
 #define TRAX_OK 0
 #define TRAX_ERR_INVALID (-1)  /* bad argument or configuration */
@@ -42,6 +42,7 @@ extern "C" {
 #define TRAX_FLAG_CAND_OVERFLOW 0x2u
 #define TRAX_FLAG_TRACKLET_OVERFLOW 0x4u
 #define TRAX_FLAG_LAYOUT 0x8u
+#define TRAX_FLAG_CONNECT_TABLE 0x10u /* connectivity table full (cannot happen at its sized capacity) */
 
 #define TRAX_MAX_LAYERS 16
 #define TRAX_MAX_TRIPLETS 8
@@ -173,6 +174,7 @@ int trax_ctx_flags(trax_ctx *ctx, void *stream, uint32_t *flags_out, int clear);
 #define TRAX_KERNEL_FILTER_MASK 5     /* filter cuts -> mask words */
 #define TRAX_KERNEL_FILTER_STORE 6    /* filter compaction */
 #define TRAX_KERNEL_SCAN 7            /* every prefix sum the context launches */
+#define TRAX_KERNEL_CONNECT 8         /* triplet connectivity: table clear, insert, lookup */
 int trax_ctx_kernel_timer(trax_ctx *ctx, int kernel);
 int trax_ctx_kernel_time(trax_ctx *ctx, double *total_ms, uint32_t *launches);
 
@@ -300,6 +302,40 @@ int trax_triplet_filter(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_e
 int trax_seed_chain(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
                     const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,
                     const trax_workspace *ws, trax_seed_outputs *out, void *stream);
+
+/* ----------------------------------------------------------- connectivity */
+/* A triplet set on the device: the hit and connectivity columns of the
+ * reference's TrackletCollection (src/datastructures/TrackletCollection.h:13-35:
+ * TrackletHit1/2/3, TrackletConnectivity), n entries each. */
+typedef struct {
+  const uint32_t *hit1, *hit2, *hit3;
+  uint32_t *connectivity; /* read only in the base set */
+  uint64_t n;
+} trax_tracklets;
+
+#define TRAX_CONNECT_BACKWARD 0x1u /* iterateBackwards */
+#define TRAX_CONNECT_TIGHT 0x2u    /* tightPacking (forward only) */
+
+/* Device workspace bytes trax_triplet_connectivity needs for n_base base
+ * triplets: a hash table of 16-byte slots, a power of two >= 2 * n_base
+ * (at least 64) plus one. */
+uint64_t trax_connectivity_workspace_bytes(uint64_t n_base);
+
+/* TripletConnectivity::run(base, following, iterateBackwards, tightPacking),
+ * src/algorithms/TripletConnectivity.h:54-56 / .cpp:21-71.  For every
+ * following triplet g, following->connectivity[g] += the sum of
+ * (base->connectivity[i] + 1) over the base triplets i joined to g (mod 2^32):
+ *   wide, forward:   base.hit3[i] == following.hit1[g]
+ *   wide, backward:  base.hit1[i] == following.hit3[g]
+ *   tight (forward): base.hit2[i] == following.hit1[g] && base.hit3[i] == following.hit2[g]
+ * as the reference's known answers state them (TripletConnectivityTest.h).
+ * Only the columns the mode reads must be set.  Tight + backward is rejected,
+ * as the reference asserts.  base and following may be the same set: every
+ * base value is read before any following value is written.  d_workspace:
+ * at least trax_connectivity_workspace_bytes(base->n) bytes, 16-byte aligned;
+ * calls that share one workspace must not overlap. */
+int trax_triplet_connectivity(trax_ctx *ctx, const trax_tracklets *base, trax_tracklets *following,
+                              uint32_t mode, void *d_workspace, uint64_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------- test support */
 /* Elementwise device evaluation of the FP32 built-in definitions used by the

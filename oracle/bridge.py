@@ -94,6 +94,9 @@ def lib():
         L.oracle_predict_window.argtypes = [C.POINTER(OracleCfg), C.c_uint32, C.c_int, _f32p, _f32p,
                                             C.c_float, C.c_float, _f32p]
         L.oracle_filter_eval.argtypes = [C.POINTER(OracleCfg), C.c_uint32, _f32p, _f32p, _f32p, _f32p]
+        L.oracle_connectivity.argtypes = [_u32p, _u32p, _u32p, _u32p, C.c_uint64, _u32p, _u32p, _u32p, _u32p,
+                                          C.c_uint64, C.c_int, C.c_int]
+        L.oracle_connectivity.restype = C.c_int
         _lib = L
     return _lib
 
@@ -253,3 +256,19 @@ def cell_of(sc, x: float, y: float, z: float):
     phi = C.c_float()
     cell = lib().oracle_cell_of(C.byref(cfg), x, y, z, C.byref(phi))
     return int(cell), phi.value
+
+
+def connectivity(base, following, backward: bool = False, tight: bool = False) -> np.ndarray:
+    """TripletConnectivity::run on host arrays.  base / following: (hit1, hit2,
+    hit3, connectivity) uint32 arrays; returns the following set's updated
+    connectivity (the inputs are not modified)."""
+    L = lib()
+    b = [np.ascontiguousarray(a, np.uint32) for a in base]
+    f = [np.ascontiguousarray(a, np.uint32) for a in following]
+    out = f[3].copy()
+    rc = L.oracle_connectivity(_p(b[0], _u32p), _p(b[1], _u32p), _p(b[2], _u32p), _p(b[3], _u32p), len(b[3]),
+                               _p(f[0], _u32p), _p(f[1], _u32p), _p(f[2], _u32p), _p(out, _u32p), len(out),
+                               int(backward), int(tight))
+    if rc != 0:
+        raise RuntimeError(f"oracle_connectivity failed ({rc})")
+    return out

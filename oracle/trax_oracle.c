@@ -759,3 +759,38 @@ void oracle_filter_eval(const oracle_cfg *c, uint32_t lt, const float *h1, const
                         const float *h3, float *out) {
   filter_pass_ex(c, lt, h1[0], h1[1], h1[2], h2[0], h2[1], h2[2], h3[0], h3[1], h3[2], out);
 }
+
+// TripletConnectivity (TripletConnectivity.cpp:21-71 selects the columns;
+// the kernels are TripletConnectivity.h:56-108).  The loop is the reference's
+// one work-item per following triplet over the base range [0, nb - 1], with
+// the join the reference's known answers fix (TripletConnectivityTest.h:
+// test_run, test_run_backward, test_run_multilayer, test_run_tight_multilayer):
+// the wide kernel as printed indexes the base column with the following
+// triplet's id and the following column with the base loop counter; that
+// order gives 2 instead of the asserted 3 on test_run_multilayer and reads out
+// of bounds when the two sets differ in size.
+int oracle_connectivity(const uint32_t *bh1, const uint32_t *bh2, const uint32_t *bh3,
+                        const uint32_t *bcon, uint64_t nb, const uint32_t *fh1, const uint32_t *fh2,
+                        const uint32_t *fh3, uint32_t *fcon, uint64_t nf, int backward, int tight) {
+  if (tight && backward) return -1;  // assert(!iterateBackwards), TripletConnectivity.cpp:56
+  if (nb == 0 || nf == 0) return 0;
+  uint32_t *w = (uint32_t *)malloc(nb * sizeof(uint32_t));
+  if (!w) return -2;
+  for (uint64_t i = 0; i < nb; ++i) w[i] = bcon[i] + 1u;
+  for (uint64_t g = 0; g < nf; ++g) {
+    uint32_t acc = fcon[g];
+    for (uint64_t i = 0; i < nb; ++i) {
+      int joined;
+      if (tight)
+        joined = bh2[i] == fh1[g] && bh3[i] == fh2[g];  // tripletConnectivityTight, :95-98
+      else if (backward)
+        joined = bh1[i] == fh3[g];
+      else
+        joined = bh3[i] == fh1[g];
+      acc += (uint32_t)joined * w[i];
+    }
+    fcon[g] = acc;
+  }
+  free(w);
+  return 0;
+}

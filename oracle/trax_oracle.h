@@ -15,6 +15,8 @@
 //   prediction      algorithms/TripletThetaPhiPredictor.h:46-248 (count),
 //                   :250-481 (store), :483-497 (offsets), .cpp:10-131
 //   filter          algorithms/TripletThetaPhiFilter.h:50-313, .cpp:10-164
+//   connectivity    algorithms/TripletConnectivity.h:56-108, .cpp:21-71
+//                   (semantics of test/TripletConnectivityTest.h)
 //
 // Where the reference is undefined or non-deterministic the oracle fixes one
 // behaviour, and the HIP path implements the same one:
@@ -114,6 +116,16 @@ void oracle_predict_window(const oracle_cfg *cfg, uint32_t lt, int store, const 
 // Filter quantities of one triplet: out = theta ratio, |dphi|, tip^2, circle radius, pass.
 void oracle_filter_eval(const oracle_cfg *cfg, uint32_t lt, const float *h1, const float *h2,
                         const float *h3, float *out);
+
+// TripletConnectivity::run (TripletConnectivity.cpp:21-71): every following
+// triplet g against every base triplet i, fcon[g] += joined * (bcon[i] + 1),
+// joined = bh3[i] == fh1[g] (wide, forward), bh1[i] == fh3[g] (wide,
+// backward), or bh2[i] == fh1[g] && bh3[i] == fh2[g] (tight, forward only).
+// The base connectivity is read from a copy taken before the first update,
+// so base and following may alias.  Returns -1 for tight + backward.
+int oracle_connectivity(const uint32_t *bh1, const uint32_t *bh2, const uint32_t *bh3,
+                        const uint32_t *bcon, uint64_t nb, const uint32_t *fh1, const uint32_t *fh2,
+                        const uint32_t *fh3, uint32_t *fcon, uint64_t nf, int backward, int tight);
 
 #ifdef __cplusplus
 }

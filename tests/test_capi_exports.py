@@ -33,7 +33,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_only_entry_points():
     lib = runtime.lib()
-    assert lib.trax_abi_version() == 3
+    assert lib.trax_abi_version() == 4
+    # connectivity table: a power of two >= 2 * n_base slots (at least 64), plus one, 16 B each
+    for n, slots in ((0, 64), (1, 64), (32, 64), (33, 128), (1000, 2048), (1 << 20, 1 << 21)):
+        assert lib.trax_connectivity_workspace_bytes(n) == (slots + 1) * 16
     for pt in (0.3, 0.5, 1.0):
         assert np.float32(lib.trax_min_radius_curvature(pt)) == ev.min_radius_curvature(pt)
     # 1 GeV at Bz = 3.8112 T: 87.52 cm (TripletConfiguration.cpp:145-152)

@@ -20,7 +20,8 @@ LIB = os.path.join(BUILD, "libtraxhip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TRAX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["scan.hip", "grid.hip", "pairs.hip", "sector.hip", "predict.hip", "filter.hip", "capi.hip"]
+DRIVERS = ["trax_seed", "trax_connect"]
+SOURCES = ["scan.hip", "grid.hip", "pairs.hip", "sector.hip", "predict.hip", "filter.hip", "connect.hip", "capi.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -56,18 +57,19 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
-    # C++ driver over the algorithm classes (host code only, links the library)
-    drv = os.path.join(BUILD, "trax_seed")
+    # C++ drivers over the algorithm classes (host code only, link the library)
     rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
-           "-I" + os.path.join(rocm, "include"), "-I" + os.path.join(ROOT, "include"),
-           os.path.join(CSRC, "host", "trax_seed.cpp"), "-L" + BUILD, "-ltraxhip",
-           "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
-           "-Wl,-rpath,$ORIGIN:" + os.path.join(rocm, "lib"), "-o", drv + ".tmp"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(drv + ".tmp", drv)
+    for name in DRIVERS:
+        drv = os.path.join(BUILD, name)
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+               "-I" + os.path.join(rocm, "include"), "-I" + os.path.join(ROOT, "include"),
+               os.path.join(CSRC, "host", name + ".cpp"), "-L" + BUILD, "-ltraxhip",
+               "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+               "-Wl,-rpath,$ORIGIN:" + os.path.join(rocm, "lib"), "-o", drv + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name} build failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(drv + ".tmp", drv)
     if verbose:
         print("built", LIB)
     return LIB

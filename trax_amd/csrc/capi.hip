@@ -260,7 +260,7 @@ extern "C" int trax_ctx_destroy(trax_ctx* ctx) {
 extern "C" int trax_ctx_kernel_timer(trax_ctx* ctx, int kernel) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
-  if (kernel < 0 || kernel > TRAX_KERNEL_SCAN) return fail(ctx, TRAX_ERR_INVALID, "unknown kernel id");
+  if (kernel < 0 || kernel > TRAX_KERNEL_CONNECT) return fail(ctx, TRAX_ERR_INVALID, "unknown kernel id");
   if (ctx->ev_start.empty()) {
     const size_t n = 256;
     for (size_t i = 0; i < n; ++i) {
@@ -745,6 +745,45 @@ hipError_t launch_fp32_eval(int op, const float* a, const float* b, float* out, 
 }
 
 }  // namespace trax
+
+// ------------------------------------------------------------ connectivity
+extern "C" uint64_t trax_connectivity_workspace_bytes(uint64_t n_base) {
+  if (n_base > (1ull << 40)) return 0;
+  return (trax::connectivity_table_slots(n_base) + 1) * sizeof(trax::ConnSlot);
+}
+
+extern "C" int trax_triplet_connectivity(trax_ctx* ctx, const trax_tracklets* base, trax_tracklets* following,
+                                         uint32_t mode, void* d_workspace, uint64_t workspace_bytes,
+                                         void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!base || !following) return fail(ctx, TRAX_ERR_INVALID, "null triplet set");
+  if (mode & ~(TRAX_CONNECT_BACKWARD | TRAX_CONNECT_TIGHT))
+    return fail(ctx, TRAX_ERR_INVALID, "unknown connectivity mode bits");
+  const bool backward = mode & TRAX_CONNECT_BACKWARD, tight = mode & TRAX_CONNECT_TIGHT;
+  // TripletConnectivity.cpp:56: assert(!iterateBackwards) for tight packing
+  if (tight && backward) return fail(ctx, TRAX_ERR_INVALID, "tight packing is forward only");
+  if (base->n > (1ull << 40)) return fail(ctx, TRAX_ERR_INVALID, "base set larger than 2^40 triplets");
+  if (base->n == 0 || following->n == 0) return TRAX_OK;
+  const bool base_cols = tight ? (base->hit2 && base->hit3) : backward ? base->hit1 != nullptr : base->hit3 != nullptr;
+  const bool follow_cols = tight ? (following->hit1 && following->hit2)
+                                 : backward ? following->hit3 != nullptr : following->hit1 != nullptr;
+  if (!base_cols || !base->connectivity) return fail(ctx, TRAX_ERR_INVALID, "base set lacks a column the mode reads");
+  if (!follow_cols || !following->connectivity)
+    return fail(ctx, TRAX_ERR_INVALID, "following set lacks a column the mode reads");
+  const uint64_t cap = trax::connectivity_table_slots(base->n);
+  if (!d_workspace || workspace_bytes < (cap + 1) * sizeof(trax::ConnSlot))
+    return fail(ctx, TRAX_ERR_INVALID, "connectivity workspace smaller than trax_connectivity_workspace_bytes");
+  if (reinterpret_cast<uintptr_t>(d_workspace) & 15u)
+    return fail(ctx, TRAX_ERR_INVALID, "connectivity workspace must be 16-byte aligned");
+  trax::ConnTriplets b{base->hit1, base->hit2, base->hit3, base->connectivity, base->n};
+  trax::ConnTriplets f{following->hit1, following->hit2, following->hit3, following->connectivity, following->n};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TRAX_TIMED(ctx, TRAX_KERNEL_CONNECT, s,
+             trax::launch_connectivity(b, f, tight ? 2 : (backward ? 1 : 0),
+                                       static_cast<trax::ConnSlot*>(d_workspace), cap, ctx->flags, s));
+  return TRAX_OK;
+}
 
 extern "C" int trax_fp32_eval(trax_ctx* ctx, int op, const float* d_a, const float* d_b, float* d_out,
                               uint64_t n, void* stream) {

@@ -15,6 +15,7 @@
 #define TRAX_FLAG_CAND_OVERFLOW 0x2u
 #define TRAX_FLAG_TRACKLET_OVERFLOW 0x4u
 #define TRAX_FLAG_LAYOUT 0x8u
+#define TRAX_FLAG_CONNECT_TABLE 0x10u
 
 namespace trax {
 

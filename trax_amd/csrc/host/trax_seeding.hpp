@@ -8,6 +8,7 @@
 //   TripletThetaPhiPredictor::run     algorithms/TripletThetaPhiPredictor.h:43-44
 //   TripletThetaPhiFilter::run        algorithms/TripletThetaPhiFilter.h:46-48
 //   PrefixSum::run                    algorithms/PrefixSum.h:42
+//   TripletConnectivity::run          algorithms/TripletConnectivity.h:54-56
 // and the data holders they exchange (HitCollection, Grid, Pairing,
 // TrackletCollection, TripletConfigurations, GeometrySupplement,
 // EventSupplement, LayerSupplement).
@@ -241,13 +242,19 @@ class Pairing {
   uint64_t count;
 };
 
-// TrackletCollection (TrackletCollection.h:13-77): hit1/2/3 + offsets.
+// TrackletCollection (TrackletCollection.h:13-77): hit1/2/3, the
+// connectivity column (zero at construction) and the offsets.
 class TrackletCollection {
  public:
   TrackletCollection(uint64_t n, uint32_t nEvents, uint32_t nLayerTriplets)
-      : hit1(n ? n : 1), hit2(n ? n : 1), hit3(n ? n : 1), trackletOffsets((size_t)nEvents * nLayerTriplets + 1),
-        count(n) {}
-  DeviceArray<uint32_t> hit1, hit2, hit3, trackletOffsets;
+      : hit1(n ? n : 1), hit2(n ? n : 1), hit3(n ? n : 1), connectivity(n ? n : 1),
+        trackletOffsets((size_t)nEvents * nLayerTriplets + 1), count(n) {
+    hip_check(hipMemset(connectivity.get(), 0, connectivity.size() * sizeof(uint32_t)), "hipMemset");
+  }
+  trax_tracklets view() const {
+    return trax_tracklets{hit1.get(), hit2.get(), hit3.get(), connectivity.get(), count};
+  }
+  DeviceArray<uint32_t> hit1, hit2, hit3, connectivity, trackletOffsets;
   uint64_t count;
 };
 
@@ -456,6 +463,28 @@ class TripletThetaPhiFilter : public KernelWrapper {
  private:
   Workspace& ws;
   Totals& tot;
+};
+
+// TripletConnectivity (TripletConnectivity.h:27-56, .cpp:21-71): adds to
+// following.connectivity, per triplet, the connectivity + 1 of every base
+// triplet whose outer hit is its inner hit (backward: whose inner hit is its
+// outer hit; tightPacking: whose two outer hits are its two inner hits).
+class TripletConnectivity : public KernelWrapper {
+ public:
+  using KernelWrapper::KernelWrapper;
+  void run(const TrackletCollection& base, TrackletCollection& following, bool iterateBackwards = false,
+           bool tightPacking = false) {
+    const uint64_t need = trax_connectivity_workspace_bytes(base.count);
+    if (need > ws_.size()) ws_.resize(need);
+    const trax_tracklets b = base.view();
+    trax_tracklets f = following.view();
+    const uint32_t mode = (iterateBackwards ? TRAX_CONNECT_BACKWARD : 0u) | (tightPacking ? TRAX_CONNECT_TIGHT : 0u);
+    ctx.check(trax_triplet_connectivity(ctx.ctx(), &b, &f, mode, ws_.get(), ws_.size(), ctx.stream()),
+              "TripletConnectivity::run");
+  }
+
+ private:
+  DeviceArray<unsigned char> ws_;
 };
 
 }  // namespace trax

@@ -89,6 +89,23 @@ hipError_t launch_filter_store(const SeedParams& sp, const uint2* pairs, const u
                                const uint32_t* scanned, uint32_t* t1, uint32_t* t2, uint32_t* t3,
                                uint64_t trk_capacity, uint32_t* trk_off, uint32_t* d_ntrk,
                                uint32_t* flags, hipStream_t stream);
+// Triplet connectivity: hash join of a base triplet set into a following one.
+struct ConnTriplets {
+  const uint32_t *hit1, *hit2, *hit3;
+  uint32_t* con;  // base: read only; following: read-modify-write
+  uint64_t n;
+};
+struct alignas(16) ConnSlot {
+  unsigned long long key;
+  uint32_t weight;
+  uint32_t pad;
+};
+// Table slots for n_base base triplets (a power of two >= 2 * n_base); the
+// table holds one more slot than that.
+uint64_t connectivity_table_slots(uint64_t n_base);
+// mode: 0 wide forward, 1 wide backward, 2 tight forward.
+hipError_t launch_connectivity(const ConnTriplets& base, const ConnTriplets& following, int mode,
+                               ConnSlot* table, uint64_t cap, uint32_t* flags, hipStream_t stream);
 hipError_t launch_fp32_eval(int op, const float* a, const float* b, float* out, uint64_t n,
                             hipStream_t stream);
 

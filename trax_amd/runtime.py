@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libtraxhip.so")
 # Kernel ids of the context's kernel timer (TRAX_KERNEL_* in include/trax_hip.h).
 KERNELS = {"grid_build": 1, "pairs": 2, "predict_walk": 3, "predict_place": 4, "filter_mask": 5,
-           "filter_store": 6, "scan": 7}
+           "filter_store": 6, "scan": 7, "connect": 8}
 # Store-window hits per pair the single-call predictor buffers before a pair walks again.
 PREDICT_BUFFER = 16
 PREDICT_TILE, PREDICT_TILE_SLOTS = 64, 512  # TRAX_PREDICT_TILE, TRAX_PREDICT_TILE_SLOTS
@@ -72,6 +72,11 @@ class WorkspaceC(C.Structure):
                 ("cand_stash", _vp), ("filter_mask", _vp), ("filter_counts", _vp)]
 
 
+class TrackletsC(C.Structure):
+    _fields_ = [("hit1", C.c_void_p), ("hit2", C.c_void_p), ("hit3", C.c_void_p),
+                ("connectivity", C.c_void_p), ("n", C.c_uint64)]
+
+
 class LaunchC(C.Structure):
     _fields_ = [("grid_waves", C.c_uint32), ("pairs_block", C.c_uint32),
                 ("predict_block", C.c_uint32), ("scan_tile", C.c_uint32)]
@@ -100,6 +105,9 @@ SIGNATURES = {
     "trax_workspace_layout": (C.c_uint64, [_P(SeedConfigC), C.c_uint32, C.c_uint32, C.c_uint64,
                                            C.c_uint64, _vp, _P(WorkspaceC)]),
     "trax_prefix_sum": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),
+    "trax_connectivity_workspace_bytes": (C.c_uint64, [C.c_uint64]),
+    "trax_triplet_connectivity": (C.c_int, [_vp, _P(TrackletsC), _P(TrackletsC), C.c_uint32, _vp, C.c_uint64,
+                                            _vp]),
     "trax_prefix_sum_dev": (C.c_int, [_vp, _vp, _vp, C.c_uint64, _vp, C.c_uint32, _vp]),
     "trax_grid_build": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
                                   _vp]),
@@ -435,6 +443,47 @@ def fetch_triplets(bufs: ChainBuffers, n_events: int, n_hits: int) -> Dict[str, 
     return {"orig_index": u(bufs.sorted["orig_index"], n_hits),
             "tracklets": np.stack([u(bufs.t[i], n_t) for i in range(3)], axis=1),
             "tracklet_offsets": u(bufs.tracklet_offsets, K1)}
+
+
+CONNECT_BACKWARD, CONNECT_TIGHT = 0x1, 0x2  # TRAX_CONNECT_*
+
+
+@dataclass
+class Tracklets:
+    """A device triplet set: hit columns and the connectivity column
+    (int32 tensors holding uint32 values); columns a mode does not read may
+    be None."""
+
+    hit1: Optional[object]
+    hit2: Optional[object]
+    hit3: Optional[object]
+    connectivity: object
+
+    @property
+    def n(self) -> int:
+        return int(self.connectivity.numel())
+
+    def c(self) -> TrackletsC:
+        return TrackletsC(_ptr(self.hit1), _ptr(self.hit2), _ptr(self.hit3), _ptr(self.connectivity), self.n)
+
+
+def connectivity_workspace(ctx: Context, n_base: int, device="cuda"):
+    """A workspace tensor for trax_triplet_connectivity over n_base base triplets."""
+    torch = _torch()
+    nbytes = int(ctx.lib.trax_connectivity_workspace_bytes(n_base))
+    return torch.empty(max(nbytes // 4, 1), dtype=torch.int32, device=device)
+
+
+def triplet_connectivity(ctx: Context, base: Tracklets, following: Tracklets, backward: bool = False,
+                         tight: bool = False, workspace=None, stream=None) -> None:
+    """TripletConnectivity::run(base, following, iterateBackwards, tightPacking):
+    adds to following.connectivity in place."""
+    ws = workspace if workspace is not None else connectivity_workspace(ctx, base.n, following.connectivity.device)
+    mode = (CONNECT_BACKWARD if backward else 0) | (CONNECT_TIGHT if tight else 0)
+    b, f = base.c(), following.c()
+    ctx.check(ctx.lib.trax_triplet_connectivity(ctx.h, C.byref(b), C.byref(f), mode, _vp(ws.data_ptr()),
+                                                ws.numel() * 4, stream_ptr(stream)),
+              "trax_triplet_connectivity")
 
 
 def prefix_sum(ctx: Context, t, stream=None) -> None:
