@@ -45,6 +45,10 @@ def parse():
                     help="short run for rocprofv3: no plain path, no CPU baseline, a 256 MiB "
                          "copy kernel for counter calibration")
     ap.add_argument("--tune", default=None, help="autotuning JSON to apply (trax_amd.autotune)")
+    ap.add_argument("--event-file", default=None,
+                    help="PEventContainer file (e.g. the reference's hits.ttBar.pb) instead of synthetic events")
+    ap.add_argument("--geometry", default=None,
+                    help="directory holding detectorRadius.dat and radiusDictionary.dat (with --event-file)")
     ap.add_argument("--pair-generator", choices=["beamspot", "sector"], default="beamspot",
                     help="pair stage: PairGeneratorBeamspot (default) or PairGeneratorSector")
     return ap.parse_args()
@@ -130,6 +134,65 @@ def cpu_baseline(sc, batch, threads: int):
     return batch.n_events / dt, sum(res), dt
 
 
+def real_events(args):
+    """A batch of real events: the container's events in file order, cycled
+    to --events (default: the file's count), selected and ordered as
+    HitCollection::addEvent does, with the reference's ttbar configuration."""
+    from trax_amd import cmsdata, pbio
+    from trax_amd import events as ev
+    if not args.geometry:
+        raise SystemExit("--event-file needs --geometry")
+    evs = pbio.load(args.event_file)
+    n = args.events or len(evs)
+    evs = [evs[i % len(evs)] for i in range(n)]
+    g = cmsdata.Geometry.load(os.path.join(args.geometry, "detectorRadius.dat"),
+                              os.path.join(args.geometry, "radiusDictionary.dat"))
+    batch, _ = cmsdata.load_events(evs, g, cmsdata.reference_triplets(), 1.0)
+    sc = cmsdata.ttbar_workload(g)
+    w = ev.Workload(name=f"cms-{os.path.splitext(os.path.basename(args.event_file))[0]}-x{n}",
+                    config_id=-1, seed_config=sc, n_events=n, radii=[])
+    log(f"{n} real events from {args.event_file}")
+    return w, batch
+
+
+def connectivity_measure(ctx, runtime, bufs, ref, n_events: int, P: int, torch):
+    """TripletConnectivity over the chain's triplets (downstream of the timed
+    step): per layer triplet p the triplets of every event form one set, and
+    tight connectivity runs p -> p+1 for p = 0..P-2, as the reference's
+    multi-layer test chains run() calls.  Algorithmic bytes per call: hit2,
+    hit3 and connectivity of each base triplet (12 B), hit1, hit2 and a
+    connectivity read + write of each following triplet (16 B)."""
+    import numpy as np
+    if P < 2:
+        return None
+    off = ref["tracklet_offsets"].astype(np.int64)
+    sets = []
+    for p in range(P):
+        idx = np.concatenate([np.arange(off[e * P + p], off[e * P + p + 1]) for e in range(n_events)])
+        di = torch.from_numpy(idx).to("cuda")
+        cols = [bufs.t[i].index_select(0, di) for i in range(3)]
+        sets.append(runtime.Tracklets(cols[0], cols[1], cols[2], torch.zeros_like(cols[0])))
+    ws = runtime.connectivity_workspace(ctx, max(t.n for t in sets[:-1]))
+
+    def run_all():
+        for p in range(P - 1):
+            runtime.triplet_connectivity(ctx, sets[p], sets[p + 1], tight=True, workspace=ws)
+    run_all()
+    torch.cuda.synchronize()
+    chains = [int((t.connectivity != 0).sum().item()) for t in sets[1:]]
+    reps = 10
+    ms, n = ctx.time_kernel("connect", run_all, reps)
+    if ctx.flags():
+        raise SystemExit("connectivity raised a device flag")
+    nb = sum(sets[p].n for p in range(P - 1))
+    nf = sum(sets[p + 1].n for p in range(P - 1))
+    calls = n / reps
+    b = (12 * nb + 16 * nf) / max(1.0, calls)
+    return {"mode": "tight, layer triplet p -> p+1", "calls": int(calls), "base_triplets": nb,
+            "following_triplets": nf, "following_with_chains": chains, "ms_per_call": ms,
+            "bytes_per_call": b, "gbs": b / (ms * 1e-3) / 1e9}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -145,13 +208,18 @@ def main():
     torch.cuda.set_device(0)
     if args.profile:
         args.no_plain = args.no_cpu = True
-    w = ev.config_workload(args.config, args.events)
+    if args.event_file:
+        w, batch = real_events(args)
+        data = f"real ({os.path.basename(args.event_file)}, events cycled to the batch size)"
+    else:
+        w = ev.config_workload(args.config, args.events)
+        log(f"generating {w.n_events} events ({w.name})")
+        batch = ev.make_batch(w)
+        data = "synthetic (seeded helix generator)"
     sc = w.seed_config
     sc.pair_generator = args.pair_generator
     if args.pair_generator != "beamspot":
         w.name += "-" + args.pair_generator
-    log(f"generating {w.n_events} events ({w.name})")
-    batch = ev.make_batch(w)
     log(f"{batch.n_hits} hits")
     ctx = runtime.Context(0)
     if args.tune:
@@ -235,6 +303,11 @@ def main():
         raise SystemExit("overflow during the per-kernel timing")
     log("kernels ms/launch: " + ", ".join(f"{k} {v['ms']:.3f}" for k, v in kern.items()))
     dominant = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["launches_per_step"])
+
+    conn = connectivity_measure(ctx, runtime, bufs, ref, batch.n_events, len(sc.triplets), torch)
+    if conn:
+        log(f"connectivity (tight, {conn['calls']} calls): {conn['ms_per_call']:.4f} ms/call, "
+            f"{conn['gbs']:.0f} GB/s algorithmic")
 
     # scan roofline on a large array (4 B read + 4 B write per element)
     n_scan = 1 << 26
@@ -321,7 +394,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+u32",
-        "data": "synthetic (seeded helix generator)",
+        "data": data,
         "config": {"workload": w.name, "events": batch.n_events, "hits": batch.n_hits,
                    "layers": sc.n_layers, "layer_triplets": len(sc.triplets),
                    "grid": f"{sc.n_phi}phi x {sc.n_z}z", "graph": not args.no_graph},
@@ -345,6 +418,7 @@ def main():
                      "unit": "GB/s", "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "ms": dom["ms"], "algorithmic_bytes": dom["bytes"]},
         "cpu_baseline": cpu,
+        "connectivity": conn,
         "copy_calibration": copy,
     }
     print(json.dumps(out), flush=True)

@@ -144,7 +144,8 @@ typedef struct {
 
 /* Launch shape of the kernels, per context (set by the autotuner). */
 typedef struct {
-  uint32_t grid_waves;    /* waves per grid-sort block: 1, 2, 4 or 8 (default 8) */
+  uint32_t grid_waves;    /* waves per grid-sort block: 1, 2, 4, 8 or 16; 0 (default) picks one
+                             per batch from the mean (event, layer) segment size */
   uint32_t pairs_block;   /* threads per block, two-phase pair kernels: 64..512 (default 256) */
   uint32_t predict_block; /* threads per block, two-phase prediction kernels (default 256) */
   uint32_t scan_tile;     /* elements per single-pass scan tile: 4096, 8192 or 16384 (default 4096) */

@@ -122,13 +122,36 @@ def test_grid_padding_row_and_column(ctx):
     _grid_parity(ctx, _config(16, 32), _batch([layers, layers], 3, rng))
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+@pytest.mark.parametrize("waves", [0, 1, 2, 4, 8, 16])
 def test_grid_block_shapes(ctx, waves):
-    """Every grid-sort block shape gives the same grid on a PU200 event."""
+    """Every grid-sort block shape (0: chosen from the mean segment size)
+    gives the same grid on a PU200 event."""
     from trax_amd import events as ev
     w = ev.config_workload(3, 2)
     try:
         ctx.set_launch(grid_waves=waves)
         _grid_parity(ctx, w.seed_config, ev.make_batch(w))
+    finally:
+        ctx.set_launch()
+
+
+@pytest.mark.parametrize("waves", [0, 1, 8])
+def test_many_small_segments_with_outliers(ctx, waves):
+    """200 events of a few hits per layer make the sort's LDS cap small
+    (1024 slots); a 900-hit segment then sorts by radix in LDS and a
+    6000-hit segment in place in the output columns, next to the tiny
+    ones, under every block shape."""
+    rng = np.random.default_rng(12)
+    events = []
+    for e in range(200):
+        layers = [_cylinder(rng, int(rng.integers(0, 9)), r) for r in (4.4, 7.3, 10.2)]
+        if e == 3:
+            layers[1] = _cylinder(rng, 900, 7.3)
+        if e == 150:
+            layers[0] = _cylinder(rng, 6000, 4.4)
+        events.append(layers)
+    try:
+        ctx.set_launch(grid_waves=waves)
+        _grid_parity(ctx, _config(16, 32), _batch(events, 3, rng))
     finally:
         ctx.set_launch()

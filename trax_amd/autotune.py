@@ -22,7 +22,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 from . import events as ev
 from . import runtime as rt
 
-WAVES = (1, 2, 4, 8)
+WAVES = (0, 1, 2, 4, 8)  # 0: chosen from the mean segment size
 BLOCKS = (64, 128, 256, 512)
 SCAN_TILES = (4096, 8192, 16384)
 PHI_BINS = (64, 128, 256)
@@ -59,7 +59,7 @@ def _time(fn, reps, torch) -> float:
 def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps: int = 3):
     """Best launch shape per stage at this binning; returns (launch, stage_ms, chain_ms)."""
     import torch
-    base = dict(grid_waves=8, pairs_block=256, predict_block=256, scan_tile=4096)
+    base = dict(grid_waves=0, pairs_block=256, predict_block=256, scan_tile=4096)
     ctx.set_launch(**base)
     bufs, db = rt.run_batch(ctx, sc, batch)
     E = db.n_events

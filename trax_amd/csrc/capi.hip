@@ -24,7 +24,7 @@ struct trax_ctx {
   uint32_t* flags = nullptr;
   unsigned long long* status = nullptr;
   size_t lds_max = 65536;
-  trax_launch_config launch = {8, 256, 256, 4096};
+  trax_launch_config launch = {0, 256, 256, 4096};
   // kernel timer: HIP events recorded on the launch stream around every
   // eager launch of one selected kernel
   int timed_kernel = 0;
@@ -292,8 +292,8 @@ extern "C" int trax_ctx_kernel_time(trax_ctx* ctx, double* total_ms, uint32_t* l
 extern "C" int trax_ctx_set_launch(trax_ctx* ctx, const trax_launch_config* cfg) {
   if (!ctx || !cfg) return TRAX_ERR_INVALID;
   const uint32_t w = cfg->grid_waves;
-  if (!(w == 1 || w == 2 || w == 4 || w == 8 || w == 16))
-    return fail(ctx, TRAX_ERR_INVALID, "grid_waves must be 1, 2, 4, 8 or 16");
+  if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16))
+    return fail(ctx, TRAX_ERR_INVALID, "grid_waves must be 0 (automatic), 1, 2, 4, 8 or 16");
   for (uint32_t b : {cfg->pairs_block, cfg->predict_block})
     if (!(b == 64 || b == 128 || b == 256 || b == 512))
       return fail(ctx, TRAX_ERR_INVALID, "block sizes must be 64, 128, 256 or 512");

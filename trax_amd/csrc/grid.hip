@@ -546,9 +546,25 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
   const size_t room = budget > kSortStaticLds ? budget - kSortStaticLds : 0u;
   uint32_t cap = (uint32_t)(room / 10u);
   cap = cap > kLdsSortMax ? kLdsSortMax : cap & ~63u;
+  // Batches of many small segments (thousands of events of a few hundred
+  // hits) get less LDS per block, so that many blocks share a CU: the cap
+  // covers the segment's cells plus four times the mean hit count (at least
+  // 1024 slots).  A larger segment takes the radix or in-place path, so the
+  // cap changes speed only.
+  uint64_t fit = (((uint64_t)sp.cells + 4ull * (gl.n_hits / nseg)) + 63u) & ~63ull;
+  fit = fit < 1024u ? 1024u : fit;
+  if (fit < cap) cap = (uint32_t)fit;
   const size_t lds = (size_t)(cap > 0 ? cap : 64u) * 10u;
   const uint32_t blocks = nseg < 65535u ? nseg : 65535u;
-  switch (gl.waves) {
+  // automatic block size: about four hits of the mean segment per thread
+  // (1 wave for the few-hundred-hit segments of single collisions, 8 for
+  // high pile-up)
+  int waves = gl.waves;
+  if (waves == 0) {
+    const uint32_t per = (gl.n_hits / nseg) / 256u;
+    waves = per >= 8u ? 8 : per >= 4u ? 4 : per >= 2u ? 2 : 1;
+  }
+  switch (waves) {
     case 1: grid_sort_launch<1>(blocks, lds, cap, stream, sp, a); break;
     case 2: grid_sort_launch<2>(blocks, lds, cap, stream, sp, a); break;
     case 4: grid_sort_launch<4>(blocks, lds, cap, stream, sp, a); break;

@@ -36,7 +36,7 @@ struct GridLaunch {
   uint32_t* grid;
   uint32_t* flags;
   size_t lds_max;  // LDS a sort block may use
-  int waves;       // waves per sort block: 1, 2, 4 or 8
+  int waves;       // waves per sort block: 1, 2, 4, 8 or 16; 0 = from the mean segment size
 };
 hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream);
 hipError_t grid_build_set_lds_limit(size_t bytes);

@@ -223,7 +223,7 @@ class Context:
             msg = self.lib.trax_ctx_last_error(self.h)
             raise TraxError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def set_launch(self, grid_waves=8, pairs_block=256, predict_block=256, scan_tile=4096):
+    def set_launch(self, grid_waves=0, pairs_block=256, predict_block=256, scan_tile=4096):
         cfg = LaunchC(grid_waves, pairs_block, predict_block, scan_tile)
         self.check(self.lib.trax_ctx_set_launch(self.h, C.byref(cfg)), "trax_ctx_set_launch")
 
