@@ -75,6 +75,7 @@ ALGO_BYTES = {
     # 7 input columns read, 9 sorted columns written, one uint32 per cell
     "grid_build": lambda c: 64 * c["hits"] + 4 * c["cells"],
     # x, y, z, phi of every hit and the cells read, one uint2 per pair written
+    # (the stage's count and store launches share these bytes)
     "pairs": lambda c: 16 * c["hits"] + 4 * c["cells"] + 8 * c["pairs"],
     # sector pairs, count + store passes: z and phi of every hit read twice,
     # its count written and read, the cells read, one uint2 per pair written

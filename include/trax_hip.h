@@ -169,7 +169,7 @@ int trax_ctx_flags(trax_ctx *ctx, void *stream, uint32_t *flags_out, int clear);
  * them and returns the summed device time and the launch count, then
  * resets.  id 0 turns the timer off. */
 #define TRAX_KERNEL_GRID_BUILD 1      /* grid build: segment sort + grid fill */
-#define TRAX_KERNEL_PAIRS 2           /* single-pass pair generation */
+#define TRAX_KERNEL_PAIRS 2           /* pair generation: the count and store kernels */
 #define TRAX_KERNEL_PREDICT_WALK 3    /* prediction window walk: counts + stash slabs */
 #define TRAX_KERNEL_PREDICT_PLACE 4   /* prediction slabs -> final candidate slots */
 #define TRAX_KERNEL_FILTER_MASK 5     /* filter cuts -> mask words */
@@ -227,9 +227,9 @@ int trax_pairs_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_even
                      const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
                      trax_seed_outputs *out, void *stream);
 
-/* Single-call form: count, resolve positions and store in one pass (no host
- * read); writes out->pairs, out->pair_offsets and out->totals[0].  Needs only
- * ws->work_offsets.  Graph-capture safe. */
+/* Single-call form: trax_pairs_count then trax_pairs_store with the total
+ * kept on the device (no host read); writes out->pairs, out->pair_offsets and
+ * out->totals[0].  Graph-capture safe. */
 int trax_pairs_beamspot(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
                         const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
                         trax_seed_outputs *out, void *stream);

@@ -445,8 +445,9 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   hipStream_t s = static_cast<hipStream_t>(stream);
   const uint32_t K = n_events * cfg->n_triplets;
   const trax::SortedHits h = sorted_view(sorted);
-  TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
-  TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
+  TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, ctx->ctrl, ctx->status, scan_sub(ctx),
+                                             s));
+  TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
                                          ws->max_hits, false, ctx->flags, ctx->launch.pairs_block, s));
   TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
                                             ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s,
@@ -468,34 +469,24 @@ extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint
   if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
   if (!sorted_ok(sorted) || !d_grid || !out || (!out->pairs && out->pair_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
-  TRAX_HIP_CHECK(ctx, trax::launch_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                          ws->pair_counts, reinterpret_cast<uint2*>(out->pairs),
                                          out->pair_capacity, ws->max_hits, true, ctx->flags,
-                                         ctx->launch.pairs_block, static_cast<hipStream_t>(stream)));
+                                         ctx->launch.pairs_block, s));
   return TRAX_OK;
 }
 
 extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
                                    const trax_hits* sorted, const uint32_t* d_grid,
                                    const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
-  int rc = check_ctx(ctx);
+  // count -> scan -> store with the total on the device.  Measured against a
+  // single pass with tile look-back, this is faster on large batches (PU200
+  // 64 events: 0.445 vs 0.554 ms; 4096 ttbar events: 0.146 vs 0.245 ms): in
+  // the single pass each tile's stores wait for its predecessors' counts.
+  int rc = trax_pairs_count(ctx, cfg, n_events, sorted, d_grid, ws, out, stream);
   if (rc) return rc;
-  trax::SeedParams sp;
-  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if (!ws || !ws->work_offsets || n_events > ws->max_events)
-    return fail(ctx, TRAX_ERR_INVALID, "workspace work offsets missing or too small");
-  if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals ||
-      (!out->pairs && out->pair_capacity))
-    return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
-  if (((uint64_t)ws->max_hits + 255) / 256 > trax::kStatusWords)
-    return fail(ctx, TRAX_ERR_INVALID, "too many hits for one launch");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s));
-  TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs_fused(sp, sorted_view(sorted), d_grid, ws->work_offsets,
-                                               reinterpret_cast<uint2*>(out->pairs), out->pair_capacity,
-                                               out->pair_offsets, out->totals + 0, ws->max_hits,
-                                               ctx->ctrl, ctx->status, ctx->flags, s));
-  return TRAX_OK;
+  return trax_pairs_store(ctx, cfg, n_events, sorted, d_grid, ws, out, stream);
 }
 
 // ------------------------------------------------------------- sector pairs
@@ -511,7 +502,8 @@ extern "C" int trax_pairs_sector_count(trax_ctx* ctx, const trax_seed_config* cf
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const uint32_t K = n_events * cfg->n_triplets;
-  TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, s, true));
+  TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, ctx->ctrl, ctx->status, scan_sub(ctx),
+                                             s, true));
   TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_sector_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                                 ws->pair_counts, nullptr, 0, ws->max_hits, false,
                                                 ctx->flags, s));

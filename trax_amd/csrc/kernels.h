@@ -41,8 +41,13 @@ struct GridLaunch {
 hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStream_t stream);
 hipError_t grid_build_set_lds_limit(size_t bytes);
 
+// Work offsets of the pair stage: per (event, layer pair) the hits of its outer
+// (inner: innermost) layer, exclusive-scanned, total at [K].  One block for
+// at most kPlanOneBlock segments, else a count kernel and the library scan.
+constexpr uint32_t kPlanOneBlock = 1024;
 hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t* work_off,
-                            hipStream_t stream, bool inner = false);
+                            ScanCtrl* ctrl, unsigned long long* status, int scan_sub, hipStream_t stream,
+                            bool inner = false);
 // Sector pairs (count or store pass over the inner-hit work list).
 hipError_t launch_sector_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                const uint32_t* work_off, uint32_t* counts, uint2* pairs,
@@ -52,11 +57,6 @@ hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_
                         const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
                         uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream);
-hipError_t launch_pairs_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                              const uint32_t* work_off, uint2* pairs, uint64_t capacity,
-                              uint32_t* pair_off, uint32_t* d_total, uint32_t max_work,
-                              ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                              hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
                                  hipStream_t stream);

@@ -119,6 +119,23 @@ __global__ __launch_bounds__(256) void k_pair_plan(SeedParams sp, const uint32_t
   if (tid == 0) work_off[K] = carry;
 }
 
+// The same counts for many segments, one per thread (work_off[K] = 0); an
+// exclusive scan of the K + 1 values then gives the offsets and the total.
+__global__ __launch_bounds__(256) void k_pair_counts(SeedParams sp, const uint32_t* __restrict__ grid,
+                                                     uint32_t* __restrict__ work_off, bool inner) {
+  const uint32_t K = sp.n_events * sp.n_triplets, C = sp.cells, L = sp.n_layers;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= K; k += gridDim.x * blockDim.x) {
+    uint32_t v = 0;
+    if (k < K) {
+      const uint32_t e = k / sp.n_triplets, p = k % sp.n_triplets;
+      const uint32_t layer = inner ? sp.layer1[p] : sp.layer2[p];
+      const uint32_t* g = grid + ((uint64_t)e * L + (layer - 1)) * C;
+      v = g[C - 1] - g[0];
+    }
+    work_off[k] = v;
+  }
+}
+
 // Inner hits of outer hit i that pass its window, in walk order: for each z
 // row of the window, the phi cells [phiLowSector, phiHighSector), wrapping
 // at the row end (PairGeneratorBeamspot.h:172-230).  Every index is visited
@@ -207,62 +224,6 @@ __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
   }
 }
 
-// Single-pass form: each 256-thread block takes ticketed tiles of 256 outer
-// hits, walks each window once to count, resolves the tile's output base by
-// decoupled look-back, walks again to store, and writes the per-segment pair
-// offsets; the last tile writes the total.
-__global__ __launch_bounds__(256) void k_pairs_fused(SeedParams sp, SortedHits h,
-                                                     const uint32_t* __restrict__ grid,
-                                                     const uint32_t* __restrict__ work_off,
-                                                     uint2* __restrict__ pairs, uint64_t capacity,
-                                                     uint32_t* __restrict__ pair_off,
-                                                     uint32_t* __restrict__ d_total, ScanCtrl* ctrl,
-                                                     unsigned long long* status,
-                                                     uint32_t* __restrict__ flags) {
-  extern __shared__ uint32_t s_offs[];
-  __shared__ uint32_t s_tile, s_excl, s_wave[4];
-  const uint32_t K = sp.n_events * sp.n_triplets;
-  const uint32_t epoch = launch_epoch(ctrl);
-  const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
-  const uint32_t W = work_off[K];
-  uint32_t ntiles = (W + 255u) / 256u;
-  if (ntiles == 0) ntiles = 1;
-  while (true) {
-    const uint32_t tile = next_tile(ctrl, &s_tile);
-    if (tile >= ntiles) break;
-    const uint32_t w = tile * 256u + threadIdx.x;
-    const bool valid = w < W;
-    PairItem it;
-    uint32_t n = 0;
-    if (valid) {
-      it = pair_item(sp, h, grid, offs, w);
-      pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
-    }
-    const TileScan ts = tile_scan_lookback(n, tile, epoch, status, s_wave, &s_excl);
-    if (valid) {
-      uint64_t pos = ts.pos;
-      bool overflow = false;
-      pair_walk(sp, it.g1, it.win, h, [&](uint32_t idx) {
-        if (pos < capacity)
-          pairs[pos] = make_uint2(idx, it.i);
-        else
-          overflow = true;
-        ++pos;
-        return false;
-      });
-      if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
-      write_segment_starts(offs, K, it.k, w, ts.pos, pair_off);
-    }
-    if (tile == ntiles - 1 && threadIdx.x == 0) {
-      const uint32_t total = ts.base + ts.total;
-      for (int64_t k = K; k >= 0 && offs[k] >= W; --k) pair_off[k] = total;
-      *d_total = total;
-    }
-    __syncthreads();  // s_tile / s_wave / s_excl are reused by the next tile
-  }
-  launch_done(ctrl, epoch);
-}
-
 // out[k] = scanned[min(idx[k], last)] for k in [0, n); total -> *d_total.
 __global__ __launch_bounds__(256) void k_gather_offsets(const uint32_t* __restrict__ scanned,
                                                         uint32_t last,
@@ -295,9 +256,19 @@ hipError_t launch_read_last(const uint32_t* arr, const uint32_t* d_index, uint32
 }
 
 hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t* work_off,
-                            hipStream_t stream, bool inner) {
-  hipLaunchKernelGGL(k_pair_plan, dim3(1), dim3(256), 0, stream, sp, grid, work_off, inner);
-  return hipGetLastError();
+                            ScanCtrl* ctrl, unsigned long long* status, int scan_sub, hipStream_t stream,
+                            bool inner) {
+  const uint32_t K = sp.n_events * sp.n_triplets;
+  if (K <= kPlanOneBlock) {  // a few block-wide scan rounds
+    hipLaunchKernelGGL(k_pair_plan, dim3(1), dim3(256), 0, stream, sp, grid, work_off, inner);
+    return hipGetLastError();
+  }
+  const uint32_t blocks = (K + 1u + 255u) / 256u;
+  hipLaunchKernelGGL(k_pair_counts, dim3(blocks < 4096u ? blocks : 4096u), dim3(256), 0, stream, sp, grid,
+                     work_off, inner);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_scan_u32(work_off, work_off, (uint64_t)K + 1u, nullptr, 0, ctrl, status, stream, scan_sub);
 }
 
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
@@ -316,18 +287,6 @@ hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_pairs_fused(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                              const uint32_t* work_off, uint2* pairs, uint64_t capacity,
-                              uint32_t* pair_off, uint32_t* d_total, uint32_t max_work,
-                              ScanCtrl* ctrl, unsigned long long* status, uint32_t* flags,
-                              hipStream_t stream) {
-  uint64_t b = ((uint64_t)max_work + 255u) / 256u;
-  if (b == 0) b = 1;
-  const uint32_t blocks = (uint32_t)(b < kPersistentBlocks ? b : kPersistentBlocks);
-  hipLaunchKernelGGL(k_pairs_fused, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                     grid, work_off, pairs, capacity, pair_off, d_total, ctrl, status, flags);
-  return hipGetLastError();
-}
 
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,

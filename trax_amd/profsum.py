@@ -37,8 +37,10 @@ from typing import Dict, List
 SHORT = [
     ("k_grid_cells", "grid_cells"),
     ("k_grid_sort", "grid_sort"),
-    ("k_pairs_fused", "pairs"),
-    ("k_pairs<", "pairs_two_phase"),
+    ("k_pairs<false>", "pairs_count"),
+    ("k_pairs<true>", "pairs_store"),
+    ("k_sector_pairs<false>", "sector_count"),
+    ("k_sector_pairs<true>", "sector_store"),
     ("k_predict_walk", "predict_walk"),
     ("k_predict_place", "predict_place"),
     ("k_predict<", "predict_two_phase"),
@@ -50,6 +52,10 @@ SHORT = [
     ("k_scan_down", "scan_down"),
     ("k_gather_offsets", "gather_offsets"),
     ("k_pair_plan", "pair_plan"),
+    ("k_pair_counts", "pair_counts"),
+    ("k_conn_clear", "connect_clear"),
+    ("k_conn_insert", "connect_insert"),
+    ("k_conn_lookup", "connect_lookup"),
     ("k_read_last", "read_last"),
 ]
 COPY_BYTES = 256 << 20  # calibration kernel: bytes read and bytes written
