@@ -108,7 +108,11 @@ __device__ __forceinline__ uint32_t segment_of(const uint32_t* __restrict__ offs
 
 // Block-wide copy of a small offsets table into LDS (n <= cap), for
 // segment_of lookups without dependent global loads.  Call from all threads.
-constexpr uint32_t kLdsOffsets = 8192;
+// Larger tables stay in global memory (L2-resident): every block copying a
+// table of thousands of entries, and the LDS it holds, cost more than the
+// binary search's dependent loads (prediction walk over 2000 ttbar events:
+// 121 us staged at 8001 entries, 41 us for 2048 events searched in place).
+constexpr uint32_t kLdsOffsets = 1024;
 
 // Dynamic LDS for the per-(event, layer triplet) offsets table of a launch.
 inline size_t offsets_lds_bytes(const SeedParams& sp) {

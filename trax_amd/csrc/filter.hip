@@ -222,11 +222,16 @@ __global__ __launch_bounds__(256) void k_filter_store(SeedParams sp, const uint2
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
   bool overflow = false;
-  for (uint32_t w = first; w < nW; w += stride) {
-    uint64_t m = mask[w];
-    uint64_t pos = scanned[w];
+  // one lane per 16 candidates: four lanes share a mask word, each starts at
+  // the word's scanned count plus the bits below its quarter, so a dense word
+  // is written by four short chains of dependent loads
+  for (uint32_t u = first; u < 4u * nW; u += stride) {
+    const uint32_t w = u >> 2, sh = (u & 3u) * 16u;
+    const uint64_t word = mask[w];
+    uint64_t m = (word >> sh) & 0xFFFFull;
+    uint64_t pos = scanned[w] + (uint32_t)__popcll(word & ((1ull << sh) - 1ull));
     while (m) {
-      const uint32_t b = (uint32_t)__builtin_ctzll(m);
+      const uint32_t b = (uint32_t)__builtin_ctzll(m) + sh;
       m &= m - 1;
       const uint2 cd = cands[w * 64u + b];
       const uint2 pr = pairs[cd.x];
@@ -272,9 +277,9 @@ hipError_t launch_filter_store(const SeedParams& sp, const uint2* pairs, const u
                                const uint32_t* scanned, uint32_t* t1, uint32_t* t2, uint32_t* t3,
                                uint64_t trk_capacity, uint32_t* trk_off, uint32_t* d_ntrk,
                                uint32_t* flags, hipStream_t stream) {
-  const uint64_t words = (cand_capacity + 63u) / 64u;
+  const uint64_t quarters = 4u * ((cand_capacity + 63u) / 64u);
   const uint64_t K = (uint64_t)sp.n_events * sp.n_triplets + 1u;
-  const uint64_t n = words > K ? words : K;
+  const uint64_t n = quarters > K ? quarters : K;
   uint64_t b = (n + 255u) / 256u;
   if (b == 0) b = 1;
   const uint32_t blocks = (uint32_t)(b < 4096u ? b : 4096u);

@@ -421,10 +421,12 @@ hipError_t launch_predict_place(const SeedParams& sp, const SortedHits& h, const
                                 uint64_t pair_capacity, const uint32_t* scanned, const uint32_t* slab,
                                 uint2* cands, uint64_t cand_capacity, uint32_t* flags,
                                 hipStream_t stream) {
-  // one tile per wave: placement is latency-bound, so launch every tile at once
+  // one tile per wave, eight rounds of a full device at most: the grid is
+  // sized from the pair capacity, which can exceed the pairs found many
+  // times over, and surplus blocks only cost launch slots
   const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
   uint64_t b = (tiles + 3) / 4;
-  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > (1u << 20) ? (1u << 20) : b));
+  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 16384u ? 16384u : b));
   hipLaunchKernelGGL(k_predict_place, dim3(blocks), dim3(256), 0, stream, sp, h, grid, pairs, pair_off,
                      d_npairs, pair_capacity, scanned, slab, cands, cand_capacity, flags);
   return hipGetLastError();
