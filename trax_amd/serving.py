@@ -25,13 +25,27 @@ from . import events as ev
 from . import runtime as rt
 
 
+HIT_ROWS = ("x", "y", "z", "layer", "det_id", "hit_id")  # rows of a request's packed hit block
+
+
 @dataclass
 class Request:
     rid: int
     arrival: float  # seconds, relative to the start of the run
     batch: ev.EventBatch  # one event
     done: float = -1.0
-    triplets: Optional[np.ndarray] = None  # (n, 3) input-hit indices within the event
+    triplets: Optional[np.ndarray] = None  # (n, 3) int32 input-hit indices within the event
+    hits: Optional[np.ndarray] = None  # [6, n] uint32: HIT_ROWS (x, y, z as float32 bits)
+    layout: Optional[np.ndarray] = None  # [2, L] uint32: per-layer hit counts and offsets
+
+    def __post_init__(self):
+        # the request's wire form: its hit columns as one contiguous block
+        if self.hits is None:
+            b = self.batch
+            self.hits = np.stack([b.x.view(np.uint32), b.y.view(np.uint32), b.z.view(np.uint32),
+                                  b.layer, b.det_id, b.hit_id]).astype(np.uint32, copy=False)
+        if self.layout is None:
+            self.layout = np.stack([self.batch.layer_nhits, self.batch.layer_offset]).astype(np.uint32)
 
 
 class Slot:
@@ -42,14 +56,25 @@ class Slot:
         self.stream = torch.cuda.Stream(device=device)
         self.ctx = rt.Context(torch.device(device).index or 0)
         self.bufs = rt.ChainBuffers(sc, max_events, max_hits, pair_cap, cand_cap, trk_cap, device)
-        n = max_hits
-        self.dev = {k: torch.empty(max(n, 1), dtype=torch.float32 if k in "xyz" else torch.int32,
-                                   device=device) for k in ("x", "y", "z", "layer", "det_id", "hit_id", "evt")}
-        self.dev_layout = {k: torch.empty(max_events * max(1, sc.n_layers), dtype=torch.int32, device=device)
-                           for k in ("event_offset", "layer_nhits", "layer_offset")}
-        self.host = {k: torch.empty_like(v, device="cpu").pin_memory() for k, v in self.dev.items()}
-        self.host_layout = {k: torch.empty_like(v, device="cpu").pin_memory()
-                            for k, v in self.dev_layout.items()}
+        n = max(max_hits, 1)
+        L = max(1, sc.n_layers)
+        # hit rows, the event column and the layout, each one block on the
+        # device and one pinned block on the host
+        self.dev_hits = torch.empty((len(HIT_ROWS), n), dtype=torch.int32, device=device)
+        self.dev_evt = torch.empty(n, dtype=torch.int32, device=device)
+        self.dev_lay = torch.empty((2, max_events * L), dtype=torch.int32, device=device)
+        self.dev_eoff = torch.empty(max_events, dtype=torch.int32, device=device)
+        self.host_hits = torch.empty((len(HIT_ROWS), n), dtype=torch.int32).pin_memory()
+        self.host_evt = torch.empty(n, dtype=torch.int32).pin_memory()
+        self.host_lay = torch.empty((2, max_events * L), dtype=torch.int32).pin_memory()
+        self.host_eoff = torch.empty(max_events, dtype=torch.int32).pin_memory()
+        self.dev = {k: (self.dev_hits[i].view(torch.float32) if k in "xyz" else self.dev_hits[i])
+                    for i, k in enumerate(HIT_ROWS)}
+        self.dev["evt"] = self.dev_evt
+        # results: triplets as input-hit indices (gathered on the device) and
+        # the per-(event, layer triplet) offsets, copied into pinned blocks
+        self.host_trk = torch.empty((0, 3), dtype=torch.int32).pin_memory()
+        self.host_toff = torch.empty(max_events * max(1, len(sc.triplets)) + 1, dtype=torch.int32).pin_memory()
         self.requests: List[Request] = []
         self.done_event = None
         self.db: Optional[rt.DeviceBatch] = None
@@ -64,44 +89,36 @@ class Slot:
         b = self.bufs
         tot = sum(t.numel() * t.element_size() for t in b.sorted.values())
         tot += b.grid.numel() * 4 + b.ws_mem.numel() + b.pairs.numel() * 4 + b.cands.numel() * 4
-        tot += sum(t.numel() * 4 for t in b.t) + sum(t.numel() * t.element_size() for t in self.dev.values())
+        tot += sum(t.numel() * 4 for t in b.t)
+        tot += 4 * (self.dev_hits.numel() + self.dev_evt.numel() + self.dev_lay.numel() + self.dev_eoff.numel())
         return int(tot)
 
     def launch(self, reqs: List[Request]) -> None:
-        """Gather the requests' hits straight into this slot's pinned staging
-        buffers (one copy per event and column, event index renumbered),
-        upload them and enqueue the chain on the slot's stream."""
+        """Gather the requests' packed hit blocks straight into this slot's
+        pinned staging blocks (one vectorised concatenation per block, event
+        index renumbered), upload them and enqueue the chain on the slot's
+        stream."""
         torch = self.torch
         L = self.bufs.sc.n_layers
-        hv = {k: t.numpy().view(np.uint32) if k not in "xyz" else t.numpy() for k, t in self.host.items()}
-        lay = {k: t.numpy().view(np.uint32) for k, t in self.host_layout.items()}
-        n = 0
-        offsets = np.zeros(len(reqs), np.int64)
-        for k, r in enumerate(reqs):
-            b = r.batch
-            m = b.n_hits
-            sl = slice(n, n + m)
-            hv["x"][sl] = b.x
-            hv["y"][sl] = b.y
-            hv["z"][sl] = b.z
-            hv["layer"][sl] = b.layer
-            hv["det_id"][sl] = b.det_id
-            hv["hit_id"][sl] = b.hit_id
-            hv["evt"][sl] = k
-            lay["event_offset"][k] = n
-            lay["layer_nhits"][k * L:(k + 1) * L] = b.layer_nhits
-            lay["layer_offset"][k * L:(k + 1) * L] = b.layer_offset
-            offsets[k] = n
-            n += m
         E = len(reqs)
+        sizes = np.fromiter((r.hits.shape[1] for r in reqs), np.int64, E)
+        offsets = np.zeros(E, np.int64)
+        np.cumsum(sizes[:-1], out=offsets[1:])
+        n = int(sizes.sum())
+        hh = self.host_hits.numpy().view(np.uint32)
+        np.concatenate([r.hits for r in reqs], axis=1, out=hh[:, :n])
+        self.host_evt.numpy()[:n] = np.repeat(np.arange(E, dtype=np.int32), sizes)
+        hl = self.host_lay.numpy().view(np.uint32)
+        np.concatenate([r.layout for r in reqs], axis=1, out=hl[:, :E * L])
+        self.host_eoff.numpy()[:E] = offsets
         with torch.cuda.stream(self.stream):
-            for k in self.dev:
-                self.dev[k][:n].copy_(self.host[k][:n], non_blocking=True)
-            for k in self.dev_layout:
-                m = E if k == "event_offset" else E * L
-                self.dev_layout[k][:m].copy_(self.host_layout[k][:m], non_blocking=True)
+            for i in range(len(HIT_ROWS)):
+                self.dev_hits[i, :n].copy_(self.host_hits[i, :n], non_blocking=True)
+            self.dev_evt[:n].copy_(self.host_evt[:n], non_blocking=True)
+            self.dev_lay[:, :E * L].copy_(self.host_lay[:, :E * L], non_blocking=True)
+            self.dev_eoff[:E].copy_(self.host_eoff[:E], non_blocking=True)
             cols = dict(self.dev)
-            cols.update(self.dev_layout)
+            cols.update(event_offset=self.dev_eoff, layer_nhits=self.dev_lay[0], layer_offset=self.dev_lay[1])
             self.db = rt.DeviceBatch(E, n, cols)
             rt.seed_chain(self.ctx, self.bufs, self.db, stream=self.stream)
             self.done_event = torch.cuda.Event()
@@ -130,13 +147,25 @@ class Slot:
             self.stream.synchronize()
             if self.ctx.flags(stream=self.stream):
                 raise rt.TraxError("slot still overflows after regrowing")
+        torch = self.torch
         P = len(self.bufs.sc.triplets)
-        with self.torch.cuda.stream(self.stream):
-            res = rt.fetch_triplets(self.bufs, len(reqs), self.n_hits)
-        offs = res["tracklet_offsets"]
-        trk = res["orig_index"][res["tracklets"]].astype(np.int64)
+        E = len(reqs)
+        n_t = int(rt.totals(self.bufs)[2])
+        if self.host_trk.shape[0] < n_t:
+            self.host_trk = torch.empty((n_t + n_t // 4 + 1024, 3), dtype=torch.int32).pin_memory()
+        with torch.cuda.stream(self.stream):
+            if n_t:
+                # input-hit index of every triplet hit, gathered on the device:
+                # only the results cross PCIe
+                idx = torch.stack([t[:n_t] for t in self.bufs.t], dim=1).long()
+                self.host_trk[:n_t].copy_(self.bufs.sorted["orig_index"][idx], non_blocking=True)
+            self.host_toff[:E * P + 1].copy_(self.bufs.tracklet_offsets[:E * P + 1], non_blocking=True)
+        self.stream.synchronize()
+        offs = self.host_toff[:E * P + 1].numpy()
+        trk = self.host_trk[:n_t].numpy()
         for e, r in enumerate(reqs):
-            r.triplets = trk[offs[e * P]:offs[(e + 1) * P]] - self.offsets[e]
+            # a new int32 array per request (the pinned block is reused)
+            r.triplets = trk[offs[e * P]:offs[(e + 1) * P]] - int(self.offsets[e])
             r.done = now
         self.requests = []
         return reqs
