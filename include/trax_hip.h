@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TRAX_ABI_VERSION 4
+#define TRAX_ABI_VERSION 5
 
 #define TRAX_OK 0
 #define TRAX_ERR_INVALID (-1)  /* bad argument or configuration */
@@ -120,6 +120,8 @@ typedef struct {
   uint32_t *cand_stash;            /* >= ceil(pair_capacity / TRAX_PREDICT_TILE) * TRAX_PREDICT_TILE_SLOTS */
   unsigned long long *filter_mask; /* >= ceil(cand_capacity / 64) + 8 */
   uint32_t *filter_counts;         /* >= ceil(cand_capacity / 64) + 9 */
+  uint32_t *cand_tile_counts;      /* >= ceil(pair_capacity / TRAX_PREDICT_TILE) + 2: candidate
+                                      slots per tile of pairs, then their scan, then the tile count */
 } trax_workspace;
 
 /* Output buffers of the pair, prediction and filter stages.  Capacities are

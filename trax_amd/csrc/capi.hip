@@ -356,6 +356,8 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
   off = align_up(off + words * 8);
   const uint64_t o_fc = off;
   off = align_up(off + (words + 1) * 4);
+  const uint64_t o_tc = off;
+  off = align_up(off + ((pair_capacity + TRAX_PREDICT_TILE - 1) / TRAX_PREDICT_TILE + 2) * 4);
   if (ws) {
     char* b = static_cast<char*>(base);
     ws->work_offsets = b ? reinterpret_cast<uint32_t*>(b + o_work) : nullptr;
@@ -368,6 +370,7 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
     ws->cand_stash = b ? reinterpret_cast<uint32_t*>(b + o_stash) : nullptr;
     ws->filter_mask = b ? reinterpret_cast<unsigned long long*>(b + o_mask) : nullptr;
     ws->filter_counts = b ? reinterpret_cast<uint32_t*>(b + o_fc) : nullptr;
+    ws->cand_tile_counts = b ? reinterpret_cast<uint32_t*>(b + o_tc) : nullptr;
   }
   return off;
 }
@@ -551,7 +554,8 @@ extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, 
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
   if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
-  if (!ws->cand_stash) return fail(ctx, TRAX_ERR_INVALID, "workspace has no predictor stash");
+  if (!ws->cand_stash || !ws->cand_tile_counts || (reinterpret_cast<uintptr_t>(ws->cand_tile_counts) & 15u))
+    return fail(ctx, TRAX_ERR_INVALID, "workspace has no predictor stash or tile counts");
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets ||
       !out->totals || (!out->pairs && out->pair_capacity) || (!out->cands && out->cand_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null prediction-stage buffer");
@@ -561,17 +565,20 @@ extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, 
   const trax::SortedHits h = sorted_view(sorted);
   const uint2* pairs = reinterpret_cast<const uint2*>(out->pairs);
   // the stash holds ceil(ws->pair_capacity / 64) slabs; pc <= ws->pair_capacity
+  // the walk leaves per-pair counts, per-tile totals and the tile count; only
+  // the tile totals are scanned (one entry per 64 pairs)
+  const uint64_t tiles_max = (pc + trax::kPredTile - 1) / trax::kPredTile;
+  uint32_t* tiles = ws->cand_tile_counts;
   TRAX_TIMED(ctx, TRAX_KERNEL_PREDICT_WALK, s,
              trax::launch_predict_walk(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
-                                       ws->cand_counts, ws->cand_stash, s));
+                                       ws->cand_counts, ws->cand_stash, tiles, s));
   TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s,
-             trax::launch_scan_u32(ws->cand_counts, ws->cand_counts, pc + 1, out->totals + 0, 1, ctx->ctrl,
+             trax::launch_scan_u32(tiles, tiles, tiles_max + 1, tiles + tiles_max + 1, 1, ctx->ctrl,
                                    ctx->status, s, scan_sub(ctx)));
-  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->cand_counts, (uint32_t)pc, out->totals + 0,
-                                                  out->pair_offsets, K + 1, out->cand_offsets,
-                                                  out->totals + 1, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_pred_offsets(tiles, ws->cand_counts, out->pair_offsets, out->totals + 0, pc,
+                                                K + 1, out->cand_offsets, out->totals + 1, s));
   TRAX_TIMED(ctx, TRAX_KERNEL_PREDICT_PLACE, s,
-             trax::launch_predict_place(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
+             trax::launch_predict_place(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc, tiles,
                                         ws->cand_counts, ws->cand_stash,
                                         reinterpret_cast<uint2*>(out->cands), out->cand_capacity,
                                         ctx->flags, s));

@@ -71,11 +71,15 @@ hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint3
 hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                                uint64_t pair_capacity, uint32_t* counts, uint32_t* slab,
-                               hipStream_t stream);
+                               uint32_t* tile_counts, hipStream_t stream);
+// Candidate offsets per segment from the scanned tile counts; total -> *d_total.
+hipError_t launch_pred_offsets(const uint32_t* tile_scanned, const uint32_t* counts, const uint32_t* pair_off,
+                               const uint32_t* d_npairs, uint64_t pair_capacity, uint32_t n,
+                               uint32_t* cand_off, uint32_t* d_total, hipStream_t stream);
 hipError_t launch_predict_place(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                 const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
-                                uint64_t pair_capacity, const uint32_t* scanned, const uint32_t* slab,
-                                uint2* cands, uint64_t cand_capacity, uint32_t* flags,
+                                uint64_t pair_capacity, const uint32_t* tile_scanned, const uint32_t* counts,
+                                const uint32_t* slab, uint2* cands, uint64_t cand_capacity, uint32_t* flags,
                                 hipStream_t stream);
 // Filter: mask pass, scan of the word counts, store pass.
 uint64_t filter_mask_words(uint64_t cand_capacity);

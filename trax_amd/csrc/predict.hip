@@ -287,7 +287,8 @@ __global__ __launch_bounds__(256) void k_predict_walk(SeedParams sp, SortedHits 
                                                       const uint32_t* __restrict__ d_npairs,
                                                       uint64_t pair_capacity,
                                                       uint32_t* __restrict__ counts,
-                                                      uint32_t* __restrict__ slab) {
+                                                      uint32_t* __restrict__ slab,
+                                                      uint32_t* __restrict__ tile_counts) {
   extern __shared__ uint32_t s_offs[];
   __shared__ uint32_t s_buf[kPredBuf * 256];  // column per thread: s_buf[j * 256 + tid]
   const uint32_t K = sp.n_events * sp.n_triplets, tid = threadIdx.x, lane = lane_id();
@@ -307,23 +308,56 @@ __global__ __launch_bounds__(256) void k_predict_walk(SeedParams sp, SortedHits 
     }
     const uint32_t incl = wave_inclusive_scan(n_count);
     const uint32_t total = __shfl(incl, 63, 64);
+    if (lane == 0) tile_counts[t] = total;
     if (total <= kTileSlots) {
       uint32_t* out = slab + (uint64_t)t * kTileSlots + (incl - n_count);
       for (uint32_t j = 0; j < n_count; ++j)
         out[j] = j < n_store ? (j < kPredBuf ? s_col[j * 256u] : kSlotRewalk) : TRAX_UNWRITTEN;
     }
   }
+  // a zero after the last tile and the tile count (the scan's length) past
+  // the largest tile count the workspace holds
+  if (blockIdx.x == 0 && tid == 0) {
+    const uint32_t tiles_max = (uint32_t)((pair_capacity + kPredTile - 1) / kPredTile);
+    tile_counts[ntiles] = 0u;
+    tile_counts[tiles_max + 1u] = ntiles;
+  }
 }
 
-// Pass 2: slab -> final slots.  `scanned` is the exclusive scan of the
-// counts over nP + 1 entries.
+// Candidate offsets per (event, layer triplet) from the tile scan: the
+// scanned slots of the tile holding the segment's first pair plus the counts
+// of that pair's tile-mates before it.  The last entry is the total.
+__global__ __launch_bounds__(256) void k_pred_offsets(const uint32_t* __restrict__ tile_scanned,
+                                                      const uint32_t* __restrict__ counts,
+                                                      const uint32_t* __restrict__ pair_off,
+                                                      const uint32_t* __restrict__ d_npairs,
+                                                      uint64_t pair_capacity, uint32_t n,
+                                                      uint32_t* __restrict__ cand_off,
+                                                      uint32_t* __restrict__ d_total) {
+  const uint64_t nP64 = *d_npairs;
+  const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    uint32_t q = pair_off[k];
+    q = q < nP ? q : nP;
+    const uint32_t t = q / kPredTile;
+    uint32_t s = tile_scanned[t];
+    for (uint32_t j = t * kPredTile; j < q; ++j) s += counts[j];
+    cand_off[k] = s;
+    if (k == n - 1u) *d_total = s;
+  }
+}
+
+// Pass 2: slab -> final slots.  `tile_scanned` is the exclusive scan of the
+// tiles' slot totals; a pair's first slot is its tile's entry plus the counts
+// of its tile-mates before it (a wave scan of `counts`).
 __global__ __launch_bounds__(256) void k_predict_place(SeedParams sp, SortedHits h,
                                                        const uint32_t* __restrict__ grid,
                                                        const uint2* __restrict__ pairs,
                                                        const uint32_t* __restrict__ pair_off,
                                                        const uint32_t* __restrict__ d_npairs,
                                                        uint64_t pair_capacity,
-                                                       const uint32_t* __restrict__ scanned,
+                                                       const uint32_t* __restrict__ tile_scanned,
+                                                       const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ slab,
                                                        uint2* __restrict__ cands,
                                                        uint64_t cand_capacity,
@@ -340,13 +374,14 @@ __global__ __launch_bounds__(256) void k_predict_place(SeedParams sp, SortedHits
   bool overflow = false;
   for (uint32_t t = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); t < ntiles; t += nwaves) {
     const uint32_t q = t * kPredTile + lane;
-    const uint32_t start = scanned[q < nP ? q : nP];  // first slot of pair q
-    const uint32_t base = __shfl(start, 0, 64);
-    const uint32_t last = t * kPredTile + kPredTile;
-    const uint32_t total = scanned[last < nP ? last : nP] - base;
+    const uint32_t n_own = q < nP ? counts[q] : 0u;  // slots of pair q
+    const uint32_t incl = wave_inclusive_scan(n_own);
+    const uint32_t base = tile_scanned[t];
+    const uint32_t start = base + incl - n_own;  // first slot of pair q
+    const uint32_t total = __shfl(incl, 63, 64);
     if (total > kTileSlots) {  // slab overflowed: every pair of the tile walks again
       if (q < nP) {
-        const uint32_t n = scanned[q + 1] - start;
+        const uint32_t n = n_own;
         if (n) {
           const PredItem it = pred_item(sp, h, grid, pairs, offs, q);
           overflow |= predict_store_walk(sp, h, it, q, start, (uint64_t)start + n, cands, cand_capacity);
@@ -357,7 +392,6 @@ __global__ __launch_bounds__(256) void k_predict_place(SeedParams sp, SortedHits
     // slot -> (owner lane, slot index within the owner's budget), through a
     // per-wave LDS map: each lane marks its own slots
     const uint32_t rel = start - base;  // this lane's first slot within the tile
-    const uint32_t n_own = (q < nP ? scanned[q + 1] : start) - start;
     uint16_t* own = s_own[threadIdx.x >> 6];
     for (uint32_t j = 0; j < n_own; ++j) own[rel + j] = (uint16_t)((j << 6) | lane);
     wave_lds_sync();
@@ -369,7 +403,7 @@ __global__ __launch_bounds__(256) void k_predict_place(SeedParams sp, SortedHits
       const uint64_t d = (uint64_t)base + sl;
       if (v == kSlotRewalk) {
         if (j == kPredBuf) {  // this pair's slots past the buffer: walk again
-          const uint32_t n = scanned[qo + 1] - scanned[qo];
+          const uint32_t n = counts[qo];
           const PredItem it = pred_item(sp, h, grid, pairs, offs, qo);
           overflow |= predict_store_walk(sp, h, it, qo, d - j, d - j + n, cands, cand_capacity,
                                          kPredBuf);
@@ -410,16 +444,26 @@ static uint32_t tile_blocks(uint64_t pair_capacity) {
 hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                                uint64_t pair_capacity, uint32_t* counts, uint32_t* slab,
-                               hipStream_t stream) {
+                               uint32_t* tile_counts, hipStream_t stream) {
   hipLaunchKernelGGL(k_predict_walk, dim3(tile_blocks(pair_capacity)), dim3(256), offsets_lds_bytes(sp),
-                     stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, slab);
+                     stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, slab, tile_counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_pred_offsets(const uint32_t* tile_scanned, const uint32_t* counts, const uint32_t* pair_off,
+                               const uint32_t* d_npairs, uint64_t pair_capacity, uint32_t n,
+                               uint32_t* cand_off, uint32_t* d_total, hipStream_t stream) {
+  uint32_t blocks = (n + 255u) / 256u;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_pred_offsets, dim3(blocks), dim3(256), 0, stream, tile_scanned, counts, pair_off,
+                     d_npairs, pair_capacity, n, cand_off, d_total);
   return hipGetLastError();
 }
 
 hipError_t launch_predict_place(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                 const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
-                                uint64_t pair_capacity, const uint32_t* scanned, const uint32_t* slab,
-                                uint2* cands, uint64_t cand_capacity, uint32_t* flags,
+                                uint64_t pair_capacity, const uint32_t* tile_scanned, const uint32_t* counts,
+                                const uint32_t* slab, uint2* cands, uint64_t cand_capacity, uint32_t* flags,
                                 hipStream_t stream) {
   // one tile per wave, eight rounds of a full device at most: the grid is
   // sized from the pair capacity, which can exceed the pairs found many
@@ -428,7 +472,7 @@ hipError_t launch_predict_place(const SeedParams& sp, const SortedHits& h, const
   uint64_t b = (tiles + 3) / 4;
   const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 16384u ? 16384u : b));
   hipLaunchKernelGGL(k_predict_place, dim3(blocks), dim3(256), 0, stream, sp, h, grid, pairs, pair_off,
-                     d_npairs, pair_capacity, scanned, slab, cands, cand_capacity, flags);
+                     d_npairs, pair_capacity, tile_scanned, counts, slab, cands, cand_capacity, flags);
   return hipGetLastError();
 }
 

@@ -69,7 +69,8 @@ class WorkspaceC(C.Structure):
     _fields_ = [("work_offsets", _vp), ("pair_counts", _vp), ("cand_counts", _vp),
                 ("max_events", C.c_uint32), ("max_hits", C.c_uint32),
                 ("pair_capacity", C.c_uint64), ("cand_capacity", C.c_uint64),
-                ("cand_stash", _vp), ("filter_mask", _vp), ("filter_counts", _vp)]
+                ("cand_stash", _vp), ("filter_mask", _vp), ("filter_counts", _vp),
+                ("cand_tile_counts", _vp)]
 
 
 class TrackletsC(C.Structure):
