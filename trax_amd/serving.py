@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import time
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Sequence
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -46,6 +46,26 @@ class Request:
                                   b.layer, b.det_id, b.hit_id]).astype(np.uint32, copy=False)
         if self.layout is None:
             self.layout = np.stack([self.batch.layer_nhits, self.batch.layer_offset]).astype(np.uint32)
+
+
+def pack_requests(reqs: Sequence["Request"], hits: np.ndarray, evt: np.ndarray, layout: np.ndarray,
+                  event_offset: np.ndarray) -> Tuple[int, np.ndarray]:
+    """Lay a batch of requests out as the chain reads it (event-major hit
+    columns, the event column renumbered 0..E-1, per-(event, layer) counts
+    and event-relative offsets, per-event offsets) in the given staging
+    blocks: hits [6, >= n] uint32, evt [>= n], layout [2, >= E * L],
+    event_offset [>= E].  Returns (n_hits, per-request first hit)."""
+    E = len(reqs)
+    sizes = np.fromiter((r.hits.shape[1] for r in reqs), np.int64, E)
+    offsets = np.zeros(E, np.int64)
+    np.cumsum(sizes[:-1], out=offsets[1:])
+    n = int(sizes.sum())
+    np.concatenate([r.hits for r in reqs], axis=1, out=hits[:, :n])
+    evt[:n] = np.repeat(np.arange(E, dtype=evt.dtype), sizes)
+    m = E * reqs[0].layout.shape[1] if E else 0
+    np.concatenate([r.layout for r in reqs], axis=1, out=layout[:, :m])
+    event_offset[:E] = offsets
+    return n, offsets
 
 
 class Slot:
@@ -101,16 +121,8 @@ class Slot:
         torch = self.torch
         L = self.bufs.sc.n_layers
         E = len(reqs)
-        sizes = np.fromiter((r.hits.shape[1] for r in reqs), np.int64, E)
-        offsets = np.zeros(E, np.int64)
-        np.cumsum(sizes[:-1], out=offsets[1:])
-        n = int(sizes.sum())
-        hh = self.host_hits.numpy().view(np.uint32)
-        np.concatenate([r.hits for r in reqs], axis=1, out=hh[:, :n])
-        self.host_evt.numpy()[:n] = np.repeat(np.arange(E, dtype=np.int32), sizes)
-        hl = self.host_lay.numpy().view(np.uint32)
-        np.concatenate([r.layout for r in reqs], axis=1, out=hl[:, :E * L])
-        self.host_eoff.numpy()[:E] = offsets
+        n, offsets = pack_requests(reqs, self.host_hits.numpy().view(np.uint32), self.host_evt.numpy(),
+                                   self.host_lay.numpy().view(np.uint32), self.host_eoff.numpy())
         with torch.cuda.stream(self.stream):
             for i in range(len(HIT_ROWS)):
                 self.dev_hits[i, :n].copy_(self.host_hits[i, :n], non_blocking=True)
