@@ -51,6 +51,7 @@ SHORT = [
     ("k_scan_partials", "scan_partials"),
     ("k_scan_down", "scan_down"),
     ("k_gather_offsets", "gather_offsets"),
+    ("k_pred_offsets", "pred_offsets"),
     ("k_pair_plan", "pair_plan"),
     ("k_pair_counts", "pair_counts"),
     ("k_conn_clear", "connect_clear"),
