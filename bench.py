@@ -119,20 +119,31 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
-def cpu_baseline(sc, batch, threads: int):
-    """The oracle chain over every event of the batch, a thread per event
-    (ctypes releases the GIL), timed on the host cores."""
+def cpu_baseline(sc, batch, threads: int, reps: int = 3, single_sample: int = 8):
+    """The oracle chain over every event of the batch, one event per task on a
+    pool of `threads` (ctypes releases the GIL): the median of `reps` runs in
+    events/s; plus one thread over the first `single_sample` events."""
     import concurrent.futures as cf
+    import statistics
 
     from oracle import bridge
     from trax_amd import events as ev
 
     singles = [ev.concat_events([batch], subset=[e]) for e in range(batch.n_events)]
+    runs, found = [], 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            res = list(ex.map(lambda b: len(bridge.run_chain(sc, b)["tracklets"]), singles))
+        runs.append(time.perf_counter() - t0)
+        found = sum(res)
+    m = min(single_sample, len(singles))
     t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-        res = list(ex.map(lambda b: len(bridge.run_chain(sc, b)["tracklets"]), singles))
-    dt = time.perf_counter() - t0
-    return batch.n_events / dt, sum(res), dt
+    for b in singles[:m]:
+        bridge.run_chain(sc, b)
+    one = m / (time.perf_counter() - t0)
+    dt = statistics.median(runs)
+    return batch.n_events / dt, found, dt, one, m
 
 
 def real_events(args):
@@ -372,14 +383,17 @@ def main():
     # ---- CPU baseline (oracle, bounded sample = this batch)
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        log(f"cpu baseline on {threads} threads")
-        cps, ctrip, cdt = cpu_baseline(sc, batch, threads)
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = max(1, min(args.cpu_threads, avail))
+        log(f"cpu baseline on {threads} threads ({avail} available)")
+        cps, ctrip, cdt, one, m = cpu_baseline(sc, batch, threads)
         if ctrip != n_trk:
             raise SystemExit(f"CPU oracle found {ctrip} triplets, GPU {n_trk}")
         cpu = {"value": cps, "unit": "events/s", "cores": threads, "kind": "port",
-               "sample": f"{batch.n_events} events of the same workload, C oracle chain, "
-                         f"{threads} threads, {cdt:.1f} s wall"}
+               "sample": f"{batch.n_events} events of the same workload, C oracle chain, one event per task "
+                         f"on {threads} threads, median of 3 runs ({cdt:.2f} s wall each)",
+               "single_thread": {"value": one, "unit": "events/s", "cores": 1,
+                                 "sample": f"the first {m} events on one thread"}}
 
     dom = kern[dominant]
     traffic = measured_traffic(w.name, dominant)
