@@ -51,6 +51,9 @@ def parse():
                     help="directory holding detectorRadius.dat and radiusDictionary.dat (with --event-file)")
     ap.add_argument("--pair-generator", choices=["beamspot", "sector"], default="beamspot",
                     help="pair stage: PairGeneratorBeamspot (default) or PairGeneratorSector")
+    ap.add_argument("--unfused", action="store_true",
+                    help="time the chain with separate prediction and filter stages (candidate "
+                         "records stored) instead of the fused prediction + filter pass")
     return ap.parse_args()
 
 
@@ -91,6 +94,13 @@ ALGO_BYTES = {
     # mask words + scanned counts read; per triplet a candidate and a pair
     # read and three hit indexes written
     "filter_store": lambda c: 12 * c["cands"] // 64 + 28 * c["triplets"],
+    # fused prediction + filter walk: the walk's bytes without the stash
+    # slots, x, y, z of each candidate's third hit read, one uint2 record
+    # per accepted triplet written
+    "predict_filter": lambda c: 16 * c["hits"] + 4 * c["cells"] + 12 * c["pairs"] + 12 * c["cands"]
+                                + 8 * c["triplets"],
+    # per triplet its record and pair read and three hit indexes written
+    "triplet_place": lambda c: 28 * c["triplets"],
 }
 
 
@@ -245,21 +255,34 @@ def main():
     n_pairs, n_cands, n_trk = runtime.totals(bufs)
     log(f"pairs {n_pairs} candidates {n_cands} triplets {n_trk}")
     ref = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+    # the fused chain (no candidate records) on its own buffers; its results
+    # must equal the staged chain's
+    fbufs, fdb = runtime.run_batch(ctx, sc, batch, candidates=False)
+    torch.cuda.synchronize()
+    fref = runtime.fetch(fbufs, batch.n_events, batch.n_hits)
+    keys = ("grid", "pairs", "pair_offsets", "cand_offsets", "tracklets", "tracklet_offsets")
+    if runtime.totals(fbufs) != (n_pairs, n_cands, n_trk) or \
+            not all(np.array_equal(fref[k], ref[k]) for k in keys):
+        raise SystemExit("fused prediction + filter differs from the staged chain")
+    if args.unfused:
+        cbufs, cdb, cref, ckeys = bufs, db, ref, keys + ("cands",)
+    else:
+        cbufs, cdb, cref, ckeys = fbufs, fdb, fref, keys
 
     # ---- batched chain, captured once as a graph and replayed
     if args.no_graph:
         def step():
-            runtime.seed_chain(ctx, bufs, db)
+            runtime.seed_chain(ctx, cbufs, cdb)
     else:
         graph = torch.cuda.CUDAGraph()
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            runtime.seed_chain(ctx, bufs, db)  # eager pass on the capture stream
+            runtime.seed_chain(ctx, cbufs, cdb)  # eager pass on the capture stream
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         with torch.cuda.graph(graph):
-            runtime.seed_chain(ctx, bufs, db)
+            runtime.seed_chain(ctx, cbufs, cdb)
         torch.cuda.synchronize()
 
         def step():
@@ -279,9 +302,8 @@ def main():
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1) / args.steps
     flags = ctx.flags()
-    got = runtime.fetch(bufs, batch.n_events, batch.n_hits)
-    same = all(np.array_equal(got[k], ref[k]) for k in ("grid", "pairs", "cands", "tracklets",
-                                                         "tracklet_offsets"))
+    got = runtime.fetch(cbufs, batch.n_events, batch.n_hits)
+    same = all(np.array_equal(got[k], cref[k]) for k in ckeys)
     if flags or not same:
         raise SystemExit(f"replayed chain differs from the eager run (flags {flags}, same {same})")
     ev_s = batch.n_events / (ms / 1e3)
@@ -294,6 +316,7 @@ def main():
     st["pairs"] = time_events(lambda: runtime.pair_stage(ctx, bufs, E), 5, torch)
     st["predict"] = time_events(lambda: runtime.triplet_predict(ctx, bufs, E), 5, torch)
     st["filter"] = time_events(lambda: runtime.triplet_filter(ctx, bufs, E), 5, torch)
+    st["predict_filter"] = time_events(lambda: runtime.triplet_predict_filter(ctx, fbufs, E), 5, torch)
     log("stages ms: " + ", ".join(f"{k} {v:.3f}" for k, v in st.items()))
 
     # ---- per-kernel device time: the library brackets every launch of the
@@ -302,9 +325,11 @@ def main():
               "cells": batch.n_events * sc.n_layers * sc.cells_per_layer}
     kern = {}
     reps = max(3, min(args.steps, 10))
-    for name in ("grid_build", "pairs", "predict_walk", "predict_place", "filter_mask",
-                 "filter_store", "scan"):
-        ms_launch, n = ctx.time_kernel(name, lambda: runtime.seed_chain(ctx, bufs, db), reps)
+    staged = ("predict_walk", "predict_place", "filter_mask", "filter_store")
+    fused = ("predict_filter", "triplet_place")
+    for name in ("grid_build", "pairs") + staged + fused + ("scan",):
+        b_, d_ = (fbufs, fdb) if name in fused else (bufs, db)
+        ms_launch, n = ctx.time_kernel(name, lambda: runtime.seed_chain(ctx, b_, d_), reps)
         kern[name] = {"ms": ms_launch, "launches_per_step": n / reps}
         key = "pairs_sector" if name == "pairs" and sc.pair_generator == "sector" else name
         if key in ALGO_BYTES:
@@ -314,7 +339,8 @@ def main():
     if ctx.flags():
         raise SystemExit("overflow during the per-kernel timing")
     log("kernels ms/launch: " + ", ".join(f"{k} {v['ms']:.3f}" for k, v in kern.items()))
-    dominant = max(kern, key=lambda k: kern[k]["ms"] * kern[k]["launches_per_step"])
+    timed = [k for k in kern if k not in (fused if args.unfused else staged)]
+    dominant = max(timed, key=lambda k: kern[k]["ms"] * kern[k]["launches_per_step"])
 
     conn = connectivity_measure(ctx, runtime, bufs, ref, batch.n_events, len(sc.triplets), torch)
     if conn:
@@ -412,7 +438,8 @@ def main():
         "data": data,
         "config": {"workload": w.name, "events": batch.n_events, "hits": batch.n_hits,
                    "layers": sc.n_layers, "layer_triplets": len(sc.triplets),
-                   "grid": f"{sc.n_phi}phi x {sc.n_z}z", "graph": not args.no_graph},
+                   "grid": f"{sc.n_phi}phi x {sc.n_z}z", "graph": not args.no_graph,
+                   "prediction_filter": "staged" if args.unfused else "fused"},
         "triplets_per_sec": n_trk / (ms / 1e3),
         "counts": {"pairs": n_pairs, "candidates": n_cands, "triplets": n_trk},
         "plain": plain,

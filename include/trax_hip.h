@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TRAX_ABI_VERSION 5
+#define TRAX_ABI_VERSION 6
 
 #define TRAX_OK 0
 #define TRAX_ERR_INVALID (-1)  /* bad argument or configuration */
@@ -122,6 +122,8 @@ typedef struct {
   uint32_t *filter_counts;         /* >= ceil(cand_capacity / 64) + 9 */
   uint32_t *cand_tile_counts;      /* >= ceil(pair_capacity / TRAX_PREDICT_TILE) + 2: candidate
                                       slots per tile of pairs, then their scan, then the tile count */
+  uint32_t *trk_tile_counts;       /* >= ceil(pair_capacity / TRAX_PREDICT_TILE) + 2: accepted
+                                      triplets per tile of pairs (trax_triplet_predict_filter) */
 } trax_workspace;
 
 /* Output buffers of the pair, prediction and filter stages.  Capacities are
@@ -178,6 +180,8 @@ int trax_ctx_flags(trax_ctx *ctx, void *stream, uint32_t *flags_out, int clear);
 #define TRAX_KERNEL_FILTER_STORE 6    /* filter compaction */
 #define TRAX_KERNEL_SCAN 7            /* every prefix sum the context launches */
 #define TRAX_KERNEL_CONNECT 8         /* triplet connectivity: table clear, insert, lookup */
+#define TRAX_KERNEL_PREDICT_FILTER 9  /* fused prediction + filter walk */
+#define TRAX_KERNEL_TRIPLET_PLACE 10  /* fused path: accepted records -> triplets */
 int trax_ctx_kernel_timer(trax_ctx *ctx, int kernel);
 int trax_ctx_kernel_time(trax_ctx *ctx, double *total_ms, uint32_t *launches);
 
@@ -296,12 +300,30 @@ int trax_triplet_filter(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_e
                         const trax_hits *sorted, const trax_workspace *ws, trax_seed_outputs *out,
                         void *stream);
 
+/* trax_triplet_predict followed by trax_triplet_filter, fused: no
+ * counterpart call in the reference, whose TripletThetaPhiPredictor::run
+ * (TripletThetaPhiPredictor.h:43-44) and TripletThetaPhiFilter::run
+ * (TripletThetaPhiFilter.h:46-48) it replaces when the candidates
+ * themselves are not wanted.  The filter's decision for a candidate depends
+ * on its three hits and layer triplet only, so one walk per pair applies the
+ * cuts to each candidate as the prediction finds it.  Writes t1/t2/t3,
+ * tracklet_offsets, cand_offsets and totals[1..2] exactly as the two calls
+ * would; out->cands is neither written nor read (cand_capacity may be 0)
+ * and no candidate overflow can occur.  Needs ws->cand_stash,
+ * ws->cand_tile_counts and ws->trk_tile_counts (16-byte aligned).  No host
+ * read; graph-capture safe. */
+int trax_triplet_predict_filter(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
+                                const trax_hits *sorted, const uint32_t *d_grid,
+                                const trax_workspace *ws, trax_seed_outputs *out, void *stream);
+
 /* ------------------------------------------------------------ whole chain */
 /* The four stages back to back with no host synchronisation (the stage
  * sequence of buildTriplets, src/triplet_builder/TripletBuilder.cpp:240-263):
  * trax_grid_build, trax_pairs_beamspot (or trax_pairs_sector, by
- * cfg->pair_generator), trax_triplet_predict, trax_triplet_filter.  Safe to capture in a hipGraph: no allocation, no
- * memset, no host read. */
+ * cfg->pair_generator), trax_triplet_predict, trax_triplet_filter; with
+ * out->cands NULL, trax_triplet_predict_filter in place of the last two (the
+ * same triplets and offsets, no candidate records).  Safe to capture in a
+ * hipGraph: no allocation, no memset, no host read. */
 int trax_seed_chain(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
                     const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,
                     const trax_workspace *ws, trax_seed_outputs *out, void *stream);

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_only_entry_points():
     lib = runtime.lib()
-    assert lib.trax_abi_version() == 5
+    assert lib.trax_abi_version() == 6
     # connectivity table: a power of two >= 2 * n_base slots (at least 64), plus one, 16 B each
     for n, slots in ((0, 64), (1, 64), (32, 64), (33, 128), (1000, 2048), (1 << 20, 1 << 21)):
         assert lib.trax_connectivity_workspace_bytes(n) == (slots + 1) * 16
@@ -53,20 +53,20 @@ def test_workspace_layout_sizes():
     need = ((4 * 2 + 1) * 4 + 10001 * 4 + 50001 * 4   # work offsets, pair and candidate counts
             + (50000 + 63) // 64 * 512 * 4            # predictor stash slabs
             + words * 8 + (words + 1) * 4             # filter mask words and their counts
-            + ((50000 + 63) // 64 + 2) * 4)           # candidate slots per tile of pairs
-    assert need <= n <= need + 7 * 256
+            + 2 * ((50000 + 63) // 64 + 2) * 4)       # candidate slots and triplets per tile of pairs
+    assert need <= n <= need + 8 * 256
     assert ws.max_events == 4 and ws.max_hits == 10000 and ws.pair_capacity == 50000
     assert ws.cand_capacity == 100000
     base = 1 << 20
     lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, 100000, ctypes.c_void_p(base),
                               ctypes.byref(ws))
     ptrs = [ws.work_offsets, ws.pair_counts, ws.cand_counts, ws.cand_stash, ws.filter_mask,
-            ws.filter_counts, ws.cand_tile_counts]
+            ws.filter_counts, ws.cand_tile_counts, ws.trk_tile_counts]
     for p in ptrs:
         assert p % 256 == 0 and base <= p < base + n
     # regions in order and disjoint
     sizes = [(4 * 2 + 1) * 4, 10001 * 4, 50001 * 4, (50000 + 63) // 64 * 512 * 4, words * 8,
-             (words + 1) * 4, ((50000 + 63) // 64 + 2) * 4]
+             (words + 1) * 4, ((50000 + 63) // 64 + 2) * 4, ((50000 + 63) // 64 + 2) * 4]
     for (p, sz), q in zip(zip(ptrs, sizes), ptrs[1:] + [base + n]):
         assert p + sz <= q
     # the grid build needs no workspace, so the binning does not change it

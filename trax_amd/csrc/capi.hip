@@ -260,7 +260,7 @@ extern "C" int trax_ctx_destroy(trax_ctx* ctx) {
 extern "C" int trax_ctx_kernel_timer(trax_ctx* ctx, int kernel) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
-  if (kernel < 0 || kernel > TRAX_KERNEL_CONNECT) return fail(ctx, TRAX_ERR_INVALID, "unknown kernel id");
+  if (kernel < 0 || kernel > TRAX_KERNEL_TRIPLET_PLACE) return fail(ctx, TRAX_ERR_INVALID, "unknown kernel id");
   if (ctx->ev_start.empty()) {
     const size_t n = 256;
     for (size_t i = 0; i < n; ++i) {
@@ -358,6 +358,8 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
   off = align_up(off + (words + 1) * 4);
   const uint64_t o_tc = off;
   off = align_up(off + ((pair_capacity + TRAX_PREDICT_TILE - 1) / TRAX_PREDICT_TILE + 2) * 4);
+  const uint64_t o_ac = off;
+  off = align_up(off + ((pair_capacity + TRAX_PREDICT_TILE - 1) / TRAX_PREDICT_TILE + 2) * 4);
   if (ws) {
     char* b = static_cast<char*>(base);
     ws->work_offsets = b ? reinterpret_cast<uint32_t*>(b + o_work) : nullptr;
@@ -371,6 +373,7 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
     ws->filter_mask = b ? reinterpret_cast<unsigned long long*>(b + o_mask) : nullptr;
     ws->filter_counts = b ? reinterpret_cast<uint32_t*>(b + o_fc) : nullptr;
     ws->cand_tile_counts = b ? reinterpret_cast<uint32_t*>(b + o_tc) : nullptr;
+    ws->trk_tile_counts = b ? reinterpret_cast<uint32_t*>(b + o_ac) : nullptr;
   }
   return off;
 }
@@ -698,6 +701,54 @@ extern "C" int trax_triplet_filter(trax_ctx* ctx, const trax_seed_config* cfg, u
   return trax_filter_store(ctx, cfg, n_events, ws, out, stream);
 }
 
+// ------------------------------------------------------ fused predict+filter
+extern "C" int trax_triplet_predict_filter(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                           const trax_hits* sorted, const uint32_t* d_grid,
+                                           const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  trax::SeedParams sp;
+  if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if (!ws->cand_stash || !ws->cand_tile_counts || !ws->trk_tile_counts ||
+      (reinterpret_cast<uintptr_t>(ws->cand_tile_counts) & 15u) ||
+      (reinterpret_cast<uintptr_t>(ws->trk_tile_counts) & 15u) || (reinterpret_cast<uintptr_t>(ws->cand_stash) & 7u))
+    return fail(ctx, TRAX_ERR_INVALID, "workspace has no predictor stash or tile counts");
+  if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets || !out->totals ||
+      !out->tracklet_offsets || (!out->pairs && out->pair_capacity) ||
+      ((!out->t1 || !out->t2 || !out->t3) && out->tracklet_capacity))
+    return fail(ctx, TRAX_ERR_INVALID, "null prediction or filter buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t K = n_events * cfg->n_triplets;
+  const uint64_t pc = pair_cap(ws, out);
+  const trax::SortedHits h = sorted_view(sorted);
+  const uint2* pairs = reinterpret_cast<const uint2*>(out->pairs);
+  const uint64_t tiles_max = (pc + trax::kPredTile - 1) / trax::kPredTile;
+  uint32_t* tiles = ws->cand_tile_counts;
+  uint32_t* acc = ws->trk_tile_counts;
+  uint2* slab = reinterpret_cast<uint2*>(ws->cand_stash);
+  TRAX_TIMED(ctx, TRAX_KERNEL_PREDICT_FILTER, s,
+             trax::launch_predict_filter_walk(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
+                                              ws->cand_counts, tiles, slab, acc, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_acc_redo(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
+                                            ws->cand_counts, slab, acc, s));
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s,
+             trax::launch_scan_u32(tiles, tiles, tiles_max + 1, tiles + tiles_max + 1, 1, ctx->ctrl,
+                                   ctx->status, s, scan_sub(ctx)));
+  TRAX_HIP_CHECK(ctx, trax::launch_pred_offsets(tiles, ws->cand_counts, out->pair_offsets, out->totals + 0, pc,
+                                                K + 1, out->cand_offsets, out->totals + 1, s));
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s,
+             trax::launch_scan_u32(acc, acc, tiles_max + 1, acc + tiles_max + 1, 1, ctx->ctrl, ctx->status, s,
+                                   scan_sub(ctx)));
+  TRAX_HIP_CHECK(ctx, trax::launch_acc_offsets(acc, slab, out->pair_offsets, out->totals + 0, pc, K + 1,
+                                               out->tracklet_offsets, out->totals + 2, s));
+  TRAX_TIMED(ctx, TRAX_KERNEL_TRIPLET_PLACE, s,
+             trax::launch_acc_place(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc, acc,
+                                    ws->cand_counts, slab, out->t1, out->t2, out->t3, out->tracklet_capacity,
+                                    ctx->flags, s));
+  return TRAX_OK;
+}
+
 // -------------------------------------------------------------- whole chain
 extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const trax_hits* in,
                                const trax_layout* layout, trax_hits* sorted, uint32_t* d_grid,
@@ -709,6 +760,7 @@ extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const
            ? trax_pairs_sector(ctx, cfg, E, sorted, d_grid, ws, out, stream)
            : trax_pairs_beamspot(ctx, cfg, E, sorted, d_grid, ws, out, stream);
   if (rc) return rc;
+  if (!out->cands) return trax_triplet_predict_filter(ctx, cfg, E, sorted, d_grid, ws, out, stream);
   if ((rc = trax_triplet_predict(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
   return trax_triplet_filter(ctx, cfg, E, sorted, ws, out, stream);
 }

@@ -16,6 +16,7 @@
 // Offsets are gathered from the scanned counts, which is what the racy
 // monotonize pass intends.
 #include "common.h"
+#include "cuts.h"
 #include "kernels.h"
 
 namespace trax {
@@ -227,9 +228,16 @@ __global__ __launch_bounds__(512) void k_predict(SeedParams sp, SortedHits h,
 constexpr uint32_t kPredBuf = 16;
 constexpr uint32_t kSlotRewalk = 0xFFFFFFFEu;
 
+// The first BUF store-window hits go to s_col[j * STRIDE]; later ones to
+// spill(j, idx).
+struct NoSpill {
+  __device__ void operator()(uint32_t, uint32_t) const {}
+};
+
+template <uint32_t BUF = kPredBuf, uint32_t STRIDE = 256, typename Spill = NoSpill>
 __device__ __forceinline__ uint32_t predict_count_buffer(const SeedParams& sp, const SortedHits& h,
                                                          const PredItem& it, uint32_t* s_col,
-                                                         uint32_t* n_store_out) {
+                                                         uint32_t* n_store_out, Spill&& spill = Spill{}) {
   const uint32_t nPhi = sp.n_phi;
   const float PI = TRAX_PI_F;
   const PredWindow wc = item_window<false>(sp, it);
@@ -248,7 +256,10 @@ __device__ __forceinline__ uint32_t predict_count_buffer(const SeedParams& sp, c
                       (wrap && ((phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= phiHigh)));
       n_count += (rc && ph && czLo <= zi && zi <= czHi) ? 1u : 0u;
       if (rs && ph && szLo <= zi && zi <= szHi) {
-        if (n_store < kPredBuf) s_col[n_store * 256u] = idx;
+        if (n_store < BUF)
+          s_col[n_store * STRIDE] = idx;
+        else
+          spill(n_store, idx);
         ++n_store;
       }
     };
@@ -447,6 +458,453 @@ hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const 
                                uint32_t* tile_counts, hipStream_t stream) {
   hipLaunchKernelGGL(k_predict_walk, dim3(tile_blocks(pair_capacity)), dim3(256), offsets_lds_bytes(sp),
                      stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, slab, tile_counts);
+  return hipGetLastError();
+}
+
+// Prediction and filter in one pass (trax_triplet_predict_filter).  The
+// filter's accept bit of a candidate is a function of its three hits and its
+// layer triplet alone, and the filter keeps candidate order; so the triplets
+// are the accepted store-window hits within each pair's slot budget, in pair
+// order and walk order, and the candidate records need not be written.
+// Pass 1 walks each tile of 64 pairs as k_predict_walk does (the count
+// budget and the store-window hits, in LDS), queues the tile's
+// buffered candidates densely in the wave's LDS and runs the cuts over the
+// queue, each cut on the previous one's survivors.  The tile's accepted
+// (pair, third hit) records go to its slab, or, when more than kAccSlots
+// are accepted, the per-pair accepted counts.  The candidate budgets still
+// give the candidate offsets and total.  After the prefix sum of the
+// accepted tile totals, pass 2 writes the triplets.
+constexpr uint32_t kAccSlots = kTileSlots / 2;  // uint2 records per tile slab
+constexpr uint32_t kTileRedo = 0xFFFFFFFFu;     // accepted total not known yet
+
+// Accepted store-window hits of pair `it` among its slots [skip, budget),
+// in walk order (slots past the store window's hits hold no hit).
+// Accepted store-window hits of pair `it` among its slots [skip, budget),
+// in walk order (slots past the store window's hits hold no hit).  The rare
+// path: the same visit order as predict_walk, one hit at a time, so the cuts
+// are inlined once rather than once per unrolled hit.
+template <typename F>
+__device__ __forceinline__ void accept_walk(const SeedParams& sp, const SortedHits& h, const PredItem& it,
+                                            uint32_t skip, uint32_t budget, F&& emit) {
+  if (skip >= budget) return;
+  const PredWindow win = item_window<true>(sp, it);
+  const uint32_t nPhi = sp.n_phi;
+  const float PI = TRAX_PI_F;
+  uint32_t seen = 0;
+  for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
+    const uint32_t row = zs * (nPhi + 1);
+    const uint32_t A = it.g3[row + win.phiLowSector];
+    const uint32_t H = it.g3[row + win.phiHighSector];
+    const uint32_t E = win.wrap ? it.g3[row + nPhi + 1] : H;  // first range [A, E)
+    const uint32_t S2 = win.wrap ? it.g3[row] : 0u;            // wrapped range [S2, H)
+    for (int part = 0; part < (win.wrap ? 2 : 1); ++part) {
+      const uint32_t lo = part ? S2 : A, hi = part ? H : E;
+      for (uint32_t idx = lo; idx < hi; ++idx) {
+        const float2 v = load_zphi(h.zphi, idx);
+        const float zi = v.x, actPhi = v.y;
+        const bool in = (win.zLow <= zi && zi <= win.zHigh) &&
+                        ((!win.wrap && (win.phiLow <= actPhi && actPhi <= win.phiHigh)) ||
+                         (win.wrap && ((win.phiLow <= actPhi && actPhi <= PI) ||
+                                       (-PI <= actPhi && actPhi <= win.phiHigh))));
+        if (!in) continue;
+        if (seen >= skip && filter_accept(sp, it.lt, it.p1x, it.p1y, it.p1z, it.p2x, it.p2y, it.p2z, h.x[idx],
+                                          h.y[idx], h.z[idx]))
+          emit(idx);
+        if (++seen >= budget) return;
+      }
+    }
+  }
+}
+
+// One cut over a wave's queue of 16-bit candidate entries; survivors are
+// compacted in place (a survivor's new position is at most its old one, and
+// the chunk is read before it is written).  With KEEP false the survivors
+// are passed to mark(entry) instead.  Idle lanes see entry `lane`, live false.
+template <bool KEEP, typename Cut, typename Mark>
+__device__ __forceinline__ uint32_t queue_cut(uint16_t* queue, uint32_t nq, Cut&& cut, Mark&& mark) {
+  const uint32_t lane = lane_id();
+  uint32_t n_out = 0;
+  for (uint32_t c0 = 0; c0 < nq; c0 += 64u) {
+    const uint32_t e = c0 + lane;
+    const uint32_t ent = e < nq ? (uint32_t)queue[e] : lane;
+    const bool ok = cut(ent, e < nq);
+    if (KEEP) {
+      const uint64_t m = __ballot(ok);
+      if (ok) queue[n_out + lanes_below(m)] = (uint16_t)ent;
+      n_out += (uint32_t)__popcll(m);
+    } else if (ok) {
+      mark(ent);
+    }
+  }
+  wave_lds_sync();
+  return n_out;
+}
+
+// Per-wave LDS of the fused walk.  A lane buffers its first kFuseBuf
+// store-window hits in its own column; later ones go to the wave's spill
+// pool (pool order is each lane's slot order).  Queue entries are
+// (j << 6 | lane) for buffered slot j, kSpillTag + k for pool entry k.
+constexpr uint32_t kFuseBuf = 8;
+constexpr uint32_t kSpill = 256;
+constexpr uint32_t kSpillTag = 1024;
+static_assert(kFuseBuf * 64 <= kSpillTag && kSpillTag + kSpill <= 65536, "queue entries are 16-bit");
+struct FuseLds {
+  uint32_t buf[kFuseBuf * 64];  // buf[j * 64 + lane]
+  uint16_t queue[kFuseBuf * 64 + kSpill];
+  uint32_t acc[64];             // accept bits of buffered slots, per lane
+  uint32_t spill_hit[kSpill];
+  uint16_t spill_tag[kSpill];   // j << 6 | lane
+  uint8_t spill_acc[kSpill];
+  uint32_t spill_n;
+};
+
+// One tile of pass 1 for the wave: pair q = t * 64 + lane.  One walk fills
+// the buffers and the slot budgets, the cuts run over the dense queue of
+// filled slots, and the tile's accepted total and slab records (or per-pair
+// counts when more than kAccSlots are accepted) are written.  Returns false,
+// leaving the tile to k_acc_redo, when the spill pool overflows.
+__device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                                           const uint2* pairs, const uint32_t* offs, uint32_t nP, uint32_t t,
+                                           FuseLds& L, uint32_t* counts, uint32_t* tile_counts, uint2* acc_slab,
+                                           uint32_t* acc_tiles) {
+  const uint32_t lane = lane_id();
+  const uint32_t q = t * kPredTile + lane;
+  uint32_t* s_col = L.buf + lane;
+  if (lane == 0) L.spill_n = 0u;
+  wave_lds_sync();
+  uint32_t n_count = 0, n_store = 0;
+  PredItem it{};
+  if (q < nP) {
+    it = pred_item(sp, h, grid, pairs, offs, q);
+    n_count = predict_count_buffer<kFuseBuf, 64>(sp, h, it, s_col, &n_store, [&](uint32_t j, uint32_t idx) {
+      const uint32_t k = atomicAdd(&L.spill_n, 1u);
+      if (k < kSpill) {
+        L.spill_hit[k] = idx;
+        L.spill_tag[k] = (uint16_t)((j << 6) | lane);
+      }
+    });
+    counts[q] = n_count;
+  }
+  const uint32_t incl = wave_inclusive_scan(n_count);
+  const uint32_t total = __shfl(incl, 63, 64);
+  if (lane == 0) tile_counts[t] = total;
+  const uint32_t n_eff = n_count < n_store ? n_count : n_store;  // slots [0, n_eff) hold a hit
+  wave_lds_sync();
+  const uint32_t ns = L.spill_n;
+  if (ns > kSpill) return false;
+  // the queue: buffered slots, then pool entries within their pair's budget
+  const uint32_t nb = n_eff < kFuseBuf ? n_eff : kFuseBuf;
+  const uint32_t qincl = wave_inclusive_scan(nb);
+  uint32_t nq = __shfl(qincl, 63, 64);
+  for (uint32_t j = 0; j < nb; ++j) L.queue[qincl - nb + j] = (uint16_t)((j << 6) | lane);
+  for (uint32_t k0 = 0; k0 < ns; k0 += 64u) {
+    const uint32_t k = k0 + lane;
+    const uint32_t tag = k < ns ? (uint32_t)L.spill_tag[k] : 0u;
+    const uint32_t ne = __shfl(n_eff, (int)(tag & 63u), 64);
+    const bool ok = k < ns && (tag >> 6) < ne;
+    const uint64_t m = __ballot(ok);
+    if (ok) L.queue[nq + lanes_below(m)] = (uint16_t)(kSpillTag + k);
+    nq += (uint32_t)__popcll(m);
+    if (k < ns) L.spill_acc[k] = 0;
+  }
+  L.acc[lane] = 0u;
+  wave_lds_sync();
+  // entry -> (owner lane, third hit); the pair's hits travel from the owner
+  // lane by shuffles
+  auto load = [&](uint32_t ent, bool live, float* v, uint32_t* i3) {
+    const bool spilled = ent >= kSpillTag;
+    const uint32_t tag = spilled ? (uint32_t)L.spill_tag[ent - kSpillTag] : ent;
+    const uint32_t l = tag & 63u;
+    v[0] = __shfl(it.p1x, (int)l, 64);
+    v[1] = __shfl(it.p1y, (int)l, 64);
+    v[2] = __shfl(it.p1z, (int)l, 64);
+    v[3] = __shfl(it.p2x, (int)l, 64);
+    v[4] = __shfl(it.p2y, (int)l, 64);
+    v[5] = __shfl(it.p2z, (int)l, 64);
+    const uint32_t lt = (uint32_t)__shfl(it.lt, (int)l, 64);
+    if (live) *i3 = spilled ? L.spill_hit[ent - kSpillTag] : L.buf[(tag >> 6) * 64u + l];
+    return lt;
+  };
+  auto none = [](uint32_t) {};
+  const uint32_t n1 = queue_cut<true>(L.queue, nq, [&](uint32_t ent, bool live) {
+    float v[6];
+    uint32_t i3 = 0;
+    const uint32_t lt = load(ent, live, v, &i3);
+    return live && filter_phi(sp, lt, v[0], v[1], v[3], v[4], h.x[i3], h.y[i3]);
+  }, none);
+  const uint32_t n2 = queue_cut<true>(L.queue, n1, [&](uint32_t ent, bool live) {
+    float v[6];
+    uint32_t i3 = 0;
+    const uint32_t lt = load(ent, live, v, &i3);
+    return live && filter_theta(sp, lt, v[0], v[1], v[2], v[3], v[4], v[5], h.x[i3], h.y[i3], h.z[i3]);
+  }, none);
+  queue_cut<false>(L.queue, n2, [&](uint32_t ent, bool live) {
+    float v[6];
+    uint32_t i3 = 0;
+    const uint32_t lt = load(ent, live, v, &i3);
+    return live && filter_circle(sp, lt, v[0], v[1], v[3], v[4], h.x[i3], h.y[i3]);
+  }, [&](uint32_t ent) {
+    if (ent >= kSpillTag)
+      L.spill_acc[ent - kSpillTag] = 1;
+    else
+      atomicOr(&L.acc[ent & 63u], 1u << (ent >> 6));
+  });
+  const uint32_t mask = L.acc[lane];
+  uint32_t a = (uint32_t)__popc(mask);
+  for (uint32_t k = 0; k < ns; ++k)  // spilled slots of this lane
+    a += (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) ? 1u : 0u;
+  const uint32_t aincl = wave_inclusive_scan(a);
+  const uint32_t A = __shfl(aincl, 63, 64);
+  if (lane == 0) acc_tiles[t] = A;
+  uint2* slab = acc_slab + (uint64_t)t * kAccSlots;
+  if (A <= kAccSlots) {
+    uint32_t pos = aincl - a;
+    for (uint32_t m = mask; m; m &= m - 1u) slab[pos++] = make_uint2(q, s_col[(uint32_t)__builtin_ctz(m) * 64u]);
+    for (uint32_t k = 0; k < ns; ++k)
+      if (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) slab[pos++] = make_uint2(q, L.spill_hit[k]);
+  } else {
+    reinterpret_cast<uint32_t*>(slab)[lane] = a;  // pass 2 walks this tile's pairs again
+  }
+  wave_lds_sync();  // the buffers, queue and pool are reused by the wave's next tile
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_predict_filter_walk(SeedParams sp, SortedHits h,
+                                                             const uint32_t* __restrict__ grid,
+                                                             const uint2* __restrict__ pairs,
+                                                             const uint32_t* __restrict__ pair_off,
+                                                             const uint32_t* __restrict__ d_npairs,
+                                                             uint64_t pair_capacity,
+                                                             uint32_t* __restrict__ counts,
+                                                             uint32_t* __restrict__ tile_counts,
+                                                             uint2* __restrict__ acc_slab,
+                                                             uint32_t* __restrict__ acc_tiles) {
+  extern __shared__ uint32_t s_offs[];
+  __shared__ FuseLds s_lds[4];
+  const uint32_t K = sp.n_events * sp.n_triplets, tid = threadIdx.x, wave = tid >> 6;
+  const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
+  const uint64_t nP64 = *d_npairs;
+  const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
+  const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
+  const uint32_t nwaves = gridDim.x * 4u;
+  for (uint32_t t = blockIdx.x * 4u + wave; t < ntiles; t += nwaves) {
+    if (!fused_tile(sp, h, grid, pairs, offs, nP, t, s_lds[wave], counts, tile_counts, acc_slab, acc_tiles) &&
+        lane_id() == 0)
+      acc_tiles[t] = kTileRedo;  // left to k_acc_redo
+  }
+  // a zero after the last tile and the tile count (the scans' length) past
+  // the largest tile count the workspace holds
+  if (blockIdx.x == 0 && tid == 0) {
+    const uint32_t tiles_max = (uint32_t)((pair_capacity + kPredTile - 1) / kPredTile);
+    tile_counts[ntiles] = 0u;
+    tile_counts[tiles_max + 1u] = ntiles;
+    acc_tiles[ntiles] = 0u;
+    acc_tiles[tiles_max + 1u] = ntiles;
+  }
+}
+
+// Tiles whose spill pool overflowed (more than kSpill slots past the lane
+// buffers; none in the PU200 and ttbar workloads): each lane walks its
+// pair's slots one by one with the cuts, once to count and once to write the
+// records (or the per-pair counts).  One lane per tile finds the marked
+// tiles; the wave then takes them in turn.
+__global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, const uint32_t* __restrict__ grid,
+                                                 const uint2* __restrict__ pairs,
+                                                 const uint32_t* __restrict__ pair_off,
+                                                 const uint32_t* __restrict__ d_npairs, uint64_t pair_capacity,
+                                                 const uint32_t* __restrict__ counts,
+                                                 uint2* __restrict__ acc_slab, uint32_t* __restrict__ acc_tiles) {
+  const uint32_t lane = lane_id();
+  const uint64_t nP64 = *d_npairs;
+  const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
+  const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
+  for (uint32_t t0 = blockIdx.x * 64u; t0 < ntiles; t0 += gridDim.x * 64u) {
+    const uint32_t tl = t0 + lane;
+    uint64_t m = __ballot(tl < ntiles && acc_tiles[tl] == kTileRedo);
+    while (m) {
+      const uint32_t t = t0 + (uint32_t)__builtin_ctzll(m);
+      m &= m - 1u;
+      const uint32_t q = t * kPredTile + lane;
+      const uint32_t n = q < nP ? counts[q] : 0u;
+      PredItem it{};
+      uint32_t a = 0;
+      if (n) {
+        it = pred_item(sp, h, grid, pairs, pair_off, q);
+        accept_walk(sp, h, it, 0u, n, [&](uint32_t) { ++a; });
+      }
+      const uint32_t aincl = wave_inclusive_scan(a);
+      const uint32_t A = __shfl(aincl, 63, 64);
+      uint2* slab = acc_slab + (uint64_t)t * kAccSlots;
+      if (A <= kAccSlots) {
+        uint32_t pos = aincl - a;
+        if (a) accept_walk(sp, h, it, 0u, n, [&](uint32_t idx) { slab[pos++] = make_uint2(q, idx); });
+      } else {
+        reinterpret_cast<uint32_t*>(slab)[lane] = a;
+      }
+      if (lane == 0) acc_tiles[t] = A;
+    }
+  }
+}
+
+// Triplet offsets per (event, layer triplet): the accepted records before
+// the segment's first pair, from the scanned tile totals plus, within that
+// pair's tile, the records of lower pairs (slab records are in pair order)
+// or the per-pair counts of an overflowed slab.  The last entry is the total.
+__global__ __launch_bounds__(256) void k_acc_offsets(const uint32_t* __restrict__ acc_scanned,
+                                                     const uint2* __restrict__ acc_slab,
+                                                     const uint32_t* __restrict__ pair_off,
+                                                     const uint32_t* __restrict__ d_npairs,
+                                                     uint64_t pair_capacity, uint32_t n,
+                                                     uint32_t* __restrict__ trk_off,
+                                                     uint32_t* __restrict__ d_total) {
+  const uint64_t nP64 = *d_npairs;
+  const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    uint32_t q = pair_off[k];
+    q = q < nP ? q : nP;
+    const uint32_t t = q / kPredTile, lq = q - t * kPredTile;
+    uint32_t s = acc_scanned[t];
+    if (lq) {  // t is a tile with pairs
+      const uint32_t A = acc_scanned[t + 1u] - s;
+      const uint2* slab = acc_slab + (uint64_t)t * kAccSlots;
+      if (A <= kAccSlots) {
+        for (uint32_t j = 0; j < A && slab[j].x < q; ++j) ++s;
+      } else {
+        const uint32_t* c = reinterpret_cast<const uint32_t*>(slab);
+        for (uint32_t l = 0; l < lq; ++l) s += c[l];
+      }
+    }
+    trk_off[k] = s;
+    if (k == n - 1u) *d_total = s;
+  }
+}
+
+// Pass 2: one lane per tile reads the tile's scanned position and accepted
+// total; the wave then copies the 64 tiles' records as one flat range
+// (record r of the wave belongs to the tile whose running total first
+// exceeds r), writing (hit1, hit2, hit3) per record.  A tile whose slab
+// overflowed walks its pairs again, the whole wave on that tile.
+__global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, const uint32_t* __restrict__ grid,
+                                                   const uint2* __restrict__ pairs,
+                                                   const uint32_t* __restrict__ pair_off,
+                                                   const uint32_t* __restrict__ d_npairs, uint64_t pair_capacity,
+                                                   const uint32_t* __restrict__ acc_scanned,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint2* __restrict__ acc_slab, uint32_t* __restrict__ t1,
+                                                   uint32_t* __restrict__ t2, uint32_t* __restrict__ t3,
+                                                   uint64_t trk_capacity, uint32_t* __restrict__ flags) {
+  __shared__ uint32_t s_incl[4][64];
+  __shared__ uint32_t s_base[4][64];
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  const uint64_t nP64 = *d_npairs;
+  const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
+  const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
+  const uint32_t nwaves = gridDim.x * 4u;
+  uint32_t* incl_w = s_incl[wave];
+  uint32_t* base_w = s_base[wave];
+  bool overflow = false;
+  for (uint32_t t0 = (blockIdx.x * 4u + wave) * 64u; t0 < ntiles; t0 += nwaves * 64u) {
+    const uint32_t t = t0 + lane;
+    uint32_t base = 0, A = 0;
+    if (t < ntiles) {
+      base = acc_scanned[t];
+      A = acc_scanned[t + 1u] - base;
+    }
+    const bool big = A > kAccSlots;
+    const uint32_t n = big ? 0u : A;
+    const uint32_t incl = wave_inclusive_scan(n);
+    const uint32_t R = __shfl(incl, 63, 64);
+    incl_w[lane] = incl;
+    base_w[lane] = base;
+    wave_lds_sync();
+    for (uint32_t r = lane; r < R; r += 64u) {
+      uint32_t lo = 0, hi = 63;  // first lane whose running total exceeds r
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (incl_w[mid] > r) hi = mid; else lo = mid + 1u;
+      }
+      const uint32_t j = r - (lo ? incl_w[lo - 1u] : 0u);
+      const uint2 rec = acc_slab[(uint64_t)(t0 + lo) * kAccSlots + j];
+      const uint2 pr = pairs[rec.x];
+      const uint64_t d = (uint64_t)base_w[lo] + j;
+      if (d < trk_capacity) {
+        t1[d] = pr.x;
+        t2[d] = pr.y;
+        t3[d] = rec.y;
+      } else {
+        overflow = true;
+      }
+    }
+    for (uint64_t m = __ballot(big); m; m &= m - 1u) {
+      const uint32_t tt = t0 + (uint32_t)__builtin_ctzll(m);
+      const uint32_t q = tt * kPredTile + lane;
+      const uint32_t a = reinterpret_cast<const uint32_t*>(acc_slab + (uint64_t)tt * kAccSlots)[lane];
+      const uint32_t ai = wave_inclusive_scan(a);
+      uint64_t d = (uint64_t)base_w[tt - t0] + ai - a;
+      if (a && q < nP) {
+        const PredItem it = pred_item(sp, h, grid, pairs, pair_off, q);
+        const uint2 pr = pairs[q];
+        accept_walk(sp, h, it, 0u, counts[q], [&](uint32_t idx) {
+          if (d < trk_capacity) {
+            t1[d] = pr.x;
+            t2[d] = pr.y;
+            t3[d] = idx;
+          } else {
+            overflow = true;
+          }
+          ++d;
+        });
+      }
+    }
+    wave_lds_sync();  // the running totals are rewritten for the wave's next tiles
+  }
+  if (overflow) atomicOr(flags, TRAX_FLAG_TRACKLET_OVERFLOW);
+}
+
+hipError_t launch_predict_filter_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
+                                      const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
+                                      uint64_t pair_capacity, uint32_t* counts, uint32_t* tile_counts,
+                                      uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream) {
+  hipLaunchKernelGGL(k_predict_filter_walk, dim3(tile_blocks(pair_capacity)), dim3(256), offsets_lds_bytes(sp),
+                     stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, tile_counts, acc_slab,
+                     acc_tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_acc_redo(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
+                           const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
+                           const uint32_t* counts, uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream) {
+  // one wave per block, one lane per tile: 64 tiles per block and round
+  const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
+  uint64_t b = (tiles + 63u) / 64u;
+  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 4096u ? 4096u : b));
+  hipLaunchKernelGGL(k_acc_redo, dim3(blocks), dim3(64), 0, stream, sp, h, grid, pairs, pair_off, d_npairs,
+                     pair_capacity, counts, acc_slab, acc_tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_acc_offsets(const uint32_t* acc_scanned, const uint2* acc_slab, const uint32_t* pair_off,
+                              const uint32_t* d_npairs, uint64_t pair_capacity, uint32_t n, uint32_t* trk_off,
+                              uint32_t* d_total, hipStream_t stream) {
+  uint32_t blocks = (n + 255u) / 256u;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_acc_offsets, dim3(blocks), dim3(256), 0, stream, acc_scanned, acc_slab, pair_off, d_npairs,
+                     pair_capacity, n, trk_off, d_total);
+  return hipGetLastError();
+}
+
+hipError_t launch_acc_place(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
+                            const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
+                            const uint32_t* acc_scanned, const uint32_t* counts, const uint2* acc_slab,
+                            uint32_t* t1, uint32_t* t2, uint32_t* t3, uint64_t trk_capacity, uint32_t* flags,
+                            hipStream_t stream) {
+  // one lane per tile: 256 tiles per block and round
+  const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
+  uint64_t b = (tiles + 255u) / 256u;
+  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 4096u ? 4096u : b));
+  hipLaunchKernelGGL(k_acc_place, dim3(blocks), dim3(256), 0, stream, sp, h, grid, pairs,
+                     pair_off, d_npairs, pair_capacity, acc_scanned, counts, acc_slab, t1, t2, t3, trk_capacity,
+                     flags);
   return hipGetLastError();
 }
 

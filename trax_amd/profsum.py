@@ -42,6 +42,10 @@ SHORT = [
     ("k_sector_pairs<false>", "sector_count"),
     ("k_sector_pairs<true>", "sector_store"),
     ("k_predict_walk", "predict_walk"),
+    ("k_predict_filter_walk", "predict_filter"),
+    ("k_acc_redo", "acc_redo"),
+    ("k_acc_place", "triplet_place"),
+    ("k_acc_offsets", "acc_offsets"),
     ("k_predict_place", "predict_place"),
     ("k_predict<", "predict_two_phase"),
     ("k_filter_mask", "filter_mask"),

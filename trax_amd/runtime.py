@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libtraxhip.so")
 # Kernel ids of the context's kernel timer (TRAX_KERNEL_* in include/trax_hip.h).
 KERNELS = {"grid_build": 1, "pairs": 2, "predict_walk": 3, "predict_place": 4, "filter_mask": 5,
-           "filter_store": 6, "scan": 7, "connect": 8}
+           "filter_store": 6, "scan": 7, "connect": 8, "predict_filter": 9, "triplet_place": 10}
 # Store-window hits per pair the single-call predictor buffers before a pair walks again.
 PREDICT_BUFFER = 16
 PREDICT_TILE, PREDICT_TILE_SLOTS = 64, 512  # TRAX_PREDICT_TILE, TRAX_PREDICT_TILE_SLOTS
@@ -70,7 +70,7 @@ class WorkspaceC(C.Structure):
                 ("max_events", C.c_uint32), ("max_hits", C.c_uint32),
                 ("pair_capacity", C.c_uint64), ("cand_capacity", C.c_uint64),
                 ("cand_stash", _vp), ("filter_mask", _vp), ("filter_counts", _vp),
-                ("cand_tile_counts", _vp)]
+                ("cand_tile_counts", _vp), ("trk_tile_counts", _vp)]
 
 
 class TrackletsC(C.Structure):
@@ -136,6 +136,8 @@ SIGNATURES = {
                                     _vp]),
     "trax_triplet_filter": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _P(WorkspaceC),
                                       _P(OutputsC), _vp]),
+    "trax_triplet_predict_filter": (C.c_int, [_vp, _P(SeedConfigC), C.c_uint32, _P(HitsC), _vp,
+                                              _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_seed_chain": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
                                   _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_fp32_eval": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_uint64, _vp]),
@@ -315,16 +317,19 @@ class DeviceBatch:
 
 
 class ChainBuffers:
-    """Resident device buffers of one chain: sorted hits, grid, workspace, outputs."""
+    """Resident device buffers of one chain: sorted hits, grid, workspace, outputs.
+    With candidates=False no candidate records are kept: the chain then runs
+    the fused prediction + filter pass (the same triplets and offsets)."""
 
     def __init__(self, sc: ev.SeedConfig, max_events: int, max_hits: int, pair_capacity: int,
-                 cand_capacity: int, tracklet_capacity: int, device="cuda"):
+                 cand_capacity: int, tracklet_capacity: int, device="cuda", candidates: bool = True):
         torch = _torch()
         self.sc = sc
         self.cfg = seed_config_c(sc)
         self.max_events, self.max_hits = int(max_events), int(max_hits)
         self.pair_capacity = int(pair_capacity)
-        self.cand_capacity = int(cand_capacity)
+        self.candidates = bool(candidates)
+        self.cand_capacity = int(cand_capacity) if candidates else 0
         self.tracklet_capacity = int(tracklet_capacity)
         n = self.max_hits
         self.sorted = {k: _empty(n, "f" if k in ("x", "y", "z", "phi") else "u", device)
@@ -342,7 +347,7 @@ class ChainBuffers:
                                 self.cand_capacity, _vp(self.ws_mem.data_ptr()), C.byref(self.ws))
         self.pairs = _empty(2 * self.pair_capacity, "u", device)
         self.pair_offsets = _empty(K1, "u", device)
-        self.cands = _empty(2 * self.cand_capacity, "u", device)
+        self.cands = _empty(2 * self.cand_capacity, "u", device) if candidates else None
         self.cand_offsets = _empty(K1, "u", device)
         self.t = [_empty(self.tracklet_capacity, "u", device) for _ in range(3)]
         self.tracklet_offsets = _empty(K1, "u", device)
@@ -355,7 +360,8 @@ class ChainBuffers:
 
     def outputs_c(self) -> OutputsC:
         return OutputsC(_ptr(self.pairs), self.pair_capacity, _ptr(self.pair_offsets),
-                        _ptr(self.cands), self.cand_capacity, _ptr(self.cand_offsets),
+                        _ptr(self.cands) if self.candidates else None, self.cand_capacity,
+                        _ptr(self.cand_offsets),
                         _ptr(self.t[0]), _ptr(self.t[1]), _ptr(self.t[2]), self.tracklet_capacity,
                         _ptr(self.tracklet_offsets), _ptr(self.totals))
 
@@ -385,13 +391,14 @@ def totals(bufs: ChainBuffers):
 
 
 def run_batch(ctx: Context, sc: ev.SeedConfig, batch: ev.EventBatch, device="cuda",
-              bufs: Optional[ChainBuffers] = None, max_retries: int = 4):
+              bufs: Optional[ChainBuffers] = None, max_retries: int = 4, candidates: bool = True):
     """Run the chain on a batch, growing the buffers until nothing overflows.
-    Returns (bufs, db)."""
+    candidates=False runs the fused prediction + filter (no candidate
+    records).  Returns (bufs, db)."""
     db = DeviceBatch.upload(batch, device)
     if bufs is None or not bufs.fits(db.n_events, db.n_hits):
         p, c, t = default_capacities(db.n_hits)
-        bufs = ChainBuffers(sc, db.n_events, max(db.n_hits, 1), p, c, t, device)
+        bufs = ChainBuffers(sc, db.n_events, max(db.n_hits, 1), p, c, t, device, candidates)
     for _ in range(max_retries):
         ctx.flags(clear=True)
         seed_chain(ctx, bufs, db)
@@ -402,7 +409,8 @@ def run_batch(ctx: Context, sc: ev.SeedConfig, batch: ev.EventBatch, device="cud
             return bufs, db
         n_p, n_c, n_t = totals(bufs)
         bufs = ChainBuffers(sc, bufs.max_events, bufs.max_hits, max(n_p, bufs.pair_capacity),
-                            max(n_c, bufs.cand_capacity), max(n_t, bufs.tracklet_capacity), device)
+                            max(n_c, bufs.cand_capacity), max(n_t, bufs.tracklet_capacity), device,
+                            bufs.candidates)
     raise TraxError("buffers still overflow after regrowing")
 
 
@@ -425,7 +433,8 @@ def fetch(bufs: ChainBuffers, n_events: int, n_hits: int) -> Dict[str, np.ndarra
         "hit_id": u(s["hit_id"], n_hits), "evt": u(s["evt"], n_hits),
         "orig_index": u(s["orig_index"], n_hits), "grid": u(bufs.grid, G),
         "pairs": u(bufs.pairs, 2 * n_p).reshape(-1, 2), "pair_offsets": u(bufs.pair_offsets, K1),
-        "cands": u(bufs.cands, 2 * n_c).reshape(-1, 2), "cand_offsets": u(bufs.cand_offsets, K1),
+        "cands": u(bufs.cands, 2 * n_c).reshape(-1, 2) if bufs.candidates else None,
+        "cand_offsets": u(bufs.cand_offsets, K1),
         "tracklets": np.stack([u(bufs.t[i], n_t) for i in range(3)], axis=1),
         "tracklet_offsets": u(bufs.tracklet_offsets, K1),
     }
@@ -579,6 +588,12 @@ def pair_stage(ctx, bufs, n_events, stream=None):
 def triplet_predict(ctx, bufs, n_events, stream=None):
     """Single-call prediction stage (walk + stash, scan, compaction; no host read)."""
     _stage(ctx, ctx.lib.trax_triplet_predict, "trax_triplet_predict", bufs, n_events, stream=stream)
+
+
+def triplet_predict_filter(ctx, bufs, n_events, stream=None):
+    """Fused prediction + filter (no candidate records; no host read)."""
+    _stage(ctx, ctx.lib.trax_triplet_predict_filter, "trax_triplet_predict_filter", bufs, n_events,
+           stream=stream)
 
 
 def filter_count(ctx, bufs, n_events, stream=None):

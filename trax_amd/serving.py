@@ -75,7 +75,10 @@ class Slot:
         self.torch = torch
         self.stream = torch.cuda.Stream(device=device)
         self.ctx = rt.Context(torch.device(device).index or 0)
-        self.bufs = rt.ChainBuffers(sc, max_events, max_hits, pair_cap, cand_cap, trk_cap, device)
+        # only the triplets are served: the chain runs the fused prediction +
+        # filter and keeps no candidate records
+        self.bufs = rt.ChainBuffers(sc, max_events, max_hits, pair_cap, cand_cap, trk_cap, device,
+                                    candidates=False)
         n = max(max_hits, 1)
         L = max(1, sc.n_layers)
         # hit rows, the event column and the layout, each one block on the
@@ -108,7 +111,8 @@ class Slot:
     def nbytes(self) -> int:
         b = self.bufs
         tot = sum(t.numel() * t.element_size() for t in b.sorted.values())
-        tot += b.grid.numel() * 4 + b.ws_mem.numel() + b.pairs.numel() * 4 + b.cands.numel() * 4
+        tot += b.grid.numel() * 4 + b.ws_mem.numel() + b.pairs.numel() * 4
+        tot += b.cands.numel() * 4 if b.cands is not None else 0
         tot += sum(t.numel() * 4 for t in b.t)
         tot += 4 * (self.dev_hits.numel() + self.dev_evt.numel() + self.dev_lay.numel() + self.dev_eoff.numel())
         return int(tot)
@@ -153,7 +157,8 @@ class Slot:
             n_p, n_c, n_t = rt.totals(self.bufs)
             b = self.bufs
             self.bufs = rt.ChainBuffers(b.sc, b.max_events, b.max_hits, max(n_p, b.pair_capacity),
-                                        max(n_c, b.cand_capacity), max(n_t, b.tracklet_capacity))
+                                        max(n_c, b.cand_capacity), max(n_t, b.tracklet_capacity),
+                                        candidates=b.candidates)
             with self.torch.cuda.stream(self.stream):
                 rt.seed_chain(self.ctx, self.bufs, self.db, stream=self.stream)
             self.stream.synchronize()
