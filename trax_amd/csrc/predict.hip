@@ -779,11 +779,14 @@ __global__ __launch_bounds__(256) void k_acc_offsets(const uint32_t* __restrict_
   }
 }
 
-// Pass 2: one lane per tile reads the tile's scanned position and accepted
-// total; the wave then copies the 64 tiles' records as one flat range
-// (record r of the wave belongs to the tile whose running total first
-// exceeds r), writing (hit1, hit2, hit3) per record.  A tile whose slab
-// overflowed walks its pairs again, the whole wave on that tile.
+// Pass 2: a block takes kPlaceTiles consecutive tiles.  Each wave reads the
+// tiles' scanned positions and accepted totals (one lane per tile) and
+// scans them; the block's threads then copy the tiles' records as one flat
+// range (record r belongs to the tile whose running total first exceeds r),
+// writing (hit1, hit2, hit3) per record.  A tile whose slab overflowed walks
+// its pairs again, one wave on that tile.
+constexpr uint32_t kPlaceTiles = 64;
+
 __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, const uint32_t* __restrict__ grid,
                                                    const uint2* __restrict__ pairs,
                                                    const uint32_t* __restrict__ pair_off,
@@ -793,20 +796,19 @@ __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, 
                                                    const uint2* __restrict__ acc_slab, uint32_t* __restrict__ t1,
                                                    uint32_t* __restrict__ t2, uint32_t* __restrict__ t3,
                                                    uint64_t trk_capacity, uint32_t* __restrict__ flags) {
-  __shared__ uint32_t s_incl[4][64];
-  __shared__ uint32_t s_base[4][64];
+  __shared__ uint32_t s_incl[4][kPlaceTiles];
+  __shared__ uint32_t s_base[4][kPlaceTiles];
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
   const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
-  const uint32_t nwaves = gridDim.x * 4u;
   uint32_t* incl_w = s_incl[wave];
   uint32_t* base_w = s_base[wave];
   bool overflow = false;
-  for (uint32_t t0 = (blockIdx.x * 4u + wave) * 64u; t0 < ntiles; t0 += nwaves * 64u) {
+  for (uint32_t t0 = blockIdx.x * kPlaceTiles; t0 < ntiles; t0 += gridDim.x * kPlaceTiles) {
     const uint32_t t = t0 + lane;
     uint32_t base = 0, A = 0;
-    if (t < ntiles) {
+    if (lane < kPlaceTiles && t < ntiles) {
       base = acc_scanned[t];
       A = acc_scanned[t + 1u] - base;
     }
@@ -814,11 +816,13 @@ __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, 
     const uint32_t n = big ? 0u : A;
     const uint32_t incl = wave_inclusive_scan(n);
     const uint32_t R = __shfl(incl, 63, 64);
-    incl_w[lane] = incl;
-    base_w[lane] = base;
+    if (lane < kPlaceTiles) {
+      incl_w[lane] = incl;
+      base_w[lane] = base;
+    }
     wave_lds_sync();
-    for (uint32_t r = lane; r < R; r += 64u) {
-      uint32_t lo = 0, hi = 63;  // first lane whose running total exceeds r
+    for (uint32_t r = wave * 64u + lane; r < R; r += 256u) {
+      uint32_t lo = 0, hi = kPlaceTiles - 1u;  // first tile whose running total exceeds r
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (incl_w[mid] > r) hi = mid; else lo = mid + 1u;
@@ -835,7 +839,9 @@ __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, 
         overflow = true;
       }
     }
-    for (uint64_t m = __ballot(big); m; m &= m - 1u) {
+    uint32_t i = 0;
+    for (uint64_t m = __ballot(big); m; m &= m - 1u, ++i) {
+      if ((i & 3u) != wave) continue;  // big tiles go round the block's waves
       const uint32_t tt = t0 + (uint32_t)__builtin_ctzll(m);
       const uint32_t q = tt * kPredTile + lane;
       const uint32_t a = reinterpret_cast<const uint32_t*>(acc_slab + (uint64_t)tt * kAccSlots)[lane];
@@ -856,7 +862,7 @@ __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, 
         });
       }
     }
-    wave_lds_sync();  // the running totals are rewritten for the wave's next tiles
+    wave_lds_sync();  // the running totals are rewritten for the next tiles
   }
   if (overflow) atomicOr(flags, TRAX_FLAG_TRACKLET_OVERFLOW);
 }
@@ -898,10 +904,10 @@ hipError_t launch_acc_place(const SeedParams& sp, const SortedHits& h, const uin
                             const uint32_t* acc_scanned, const uint32_t* counts, const uint2* acc_slab,
                             uint32_t* t1, uint32_t* t2, uint32_t* t3, uint64_t trk_capacity, uint32_t* flags,
                             hipStream_t stream) {
-  // one lane per tile: 256 tiles per block and round
+  // kPlaceTiles tiles per block and round
   const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
-  uint64_t b = (tiles + 255u) / 256u;
-  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 4096u ? 4096u : b));
+  uint64_t b = (tiles + kPlaceTiles - 1u) / kPlaceTiles;
+  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 16384u ? 16384u : b));
   hipLaunchKernelGGL(k_acc_place, dim3(blocks), dim3(256), 0, stream, sp, h, grid, pairs,
                      pair_off, d_npairs, pair_capacity, acc_scanned, counts, acc_slab, t1, t2, t3, trk_capacity,
                      flags);
