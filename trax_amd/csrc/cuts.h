@@ -13,15 +13,20 @@ namespace trax {
 // circle-fit TIP and radius cuts.  The reference evaluates all of them and
 // ANDs the results; the cuts are pure functions, so evaluating them in any
 // order, each only on the previous one's survivors, gives the same product.
-__device__ __forceinline__ bool filter_phi(const SeedParams& sp, uint32_t lt, float x1, float y1,
-                                           float x2, float y2, float x3, float y3) {
+// The phi cut with the pair's own angle atan2(y2 - y1, x2 - x1) given.
+__device__ __forceinline__ bool filter_phi_a1(const SeedParams& sp, uint32_t lt, float angle1, float x2,
+                                              float y2, float x3, float y3) {
   const float dPhiCut = sp.dphi_cut[lt];
   const float PI = TRAX_PI_F;
-  const float angle1 = trax_atan2f((y2 - y1), (x2 - x1));
   const float angle2 = trax_atan2f((y3 - y2), (x3 - x2));
   float delta = angle2 - angle1;
   delta += (delta > PI) ? -2 * PI : (delta < -PI) ? 2 * PI : 0;
   return __builtin_fabsf(delta) <= dPhiCut;
+}
+
+__device__ __forceinline__ bool filter_phi(const SeedParams& sp, uint32_t lt, float x1, float y1,
+                                           float x2, float y2, float x3, float y3) {
+  return filter_phi_a1(sp, lt, trax_atan2f((y2 - y1), (x2 - x1)), x2, y2, x3, y3);
 }
 
 __device__ __forceinline__ bool filter_theta(const SeedParams& sp, uint32_t lt, float x1, float y1,

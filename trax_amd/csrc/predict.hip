@@ -626,11 +626,18 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
     return lt;
   };
   auto none = [](uint32_t) {};
+  // the phi cut's pair angle, once per pair
+  const float a1 = nq ? trax_atan2f((it.p2y - it.p1y), (it.p2x - it.p1x)) : 0.0f;
   const uint32_t n1 = queue_cut<true>(L.queue, nq, [&](uint32_t ent, bool live) {
-    float v[6];
-    uint32_t i3 = 0;
-    const uint32_t lt = load(ent, live, v, &i3);
-    return live && filter_phi(sp, lt, v[0], v[1], v[3], v[4], h.x[i3], h.y[i3]);
+    const bool spilled = ent >= kSpillTag;
+    const uint32_t tag = spilled ? (uint32_t)L.spill_tag[ent - kSpillTag] : ent;
+    const uint32_t l = tag & 63u;
+    const float ang = __shfl(a1, (int)l, 64);
+    const float x2 = __shfl(it.p2x, (int)l, 64), y2 = __shfl(it.p2y, (int)l, 64);
+    const uint32_t lt = (uint32_t)__shfl(it.lt, (int)l, 64);
+    if (!live) return false;
+    const uint32_t i3 = spilled ? L.spill_hit[ent - kSpillTag] : L.buf[(tag >> 6) * 64u + l];
+    return filter_phi_a1(sp, lt, ang, x2, y2, h.x[i3], h.y[i3]);
   }, none);
   const uint32_t n2 = queue_cut<true>(L.queue, n1, [&](uint32_t ent, bool live) {
     float v[6];
