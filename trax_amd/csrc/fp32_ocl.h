@@ -42,13 +42,14 @@ TRAX_FP float trax_atan_poly(float u) {
   return (p * z) * u + u;
 }
 
-// atan(t) for t in [0, 1]
+// atan(t) for t in [0, 1].  Both reductions are evaluated with one division
+// and one polynomial (operands chosen by selects), so a wave whose lanes take
+// different branches pays for one of each.
 TRAX_FP float trax_atan_unit(float t) {
-  if (t > TRAX_TAN_PI8) {
-    float u = (t - 1.0f) / (t + 1.0f);
-    return TRAX_PIO4_F + (trax_atan_poly(u) + TRAX_PIO4_LO);
-  }
-  return trax_atan_poly(t);
+  const int big = t > TRAX_TAN_PI8;
+  const float u = big ? (t - 1.0f) / (t + 1.0f) : t;  // the division is evaluated either way
+  const float p = trax_atan_poly(u);
+  return big ? TRAX_PIO4_F + (p + TRAX_PIO4_LO) : p;
 }
 
 TRAX_FP float trax_copysign(float mag, float sgn) { return __builtin_copysignf(mag, sgn); }
@@ -56,11 +57,11 @@ TRAX_FP float trax_copysign(float mag, float sgn) { return __builtin_copysignf(m
 TRAX_FP float trax_atanf(float x) {
   if (x != x) return x;
   float a = __builtin_fabsf(x);
-  float r;
-  if (a > 1.0f)
-    r = (TRAX_PIO2_F - trax_atan_unit(1.0f / a)) + TRAX_PIO2_LO;
-  else
-    r = trax_atan_unit(a);
+  // atan(1/a) reduction for a > 1, through one call (operand by select)
+  const int inv = a > 1.0f;
+  const float t = inv ? 1.0f / a : a;  // the division is evaluated either way
+  const float u = trax_atan_unit(t);
+  const float r = inv ? (TRAX_PIO2_F - u) + TRAX_PIO2_LO : u;
   return trax_copysign(r, x);
 }
 
@@ -82,10 +83,11 @@ TRAX_FP float trax_atan2f(float y, float x) {
     else
       r = xneg ? TRAX_PI_F : 0.0f;
   } else {
-    if (ay > ax)
-      r = (TRAX_PIO2_F - trax_atan_unit(ax / ay)) + TRAX_PIO2_LO;
-    else
-      r = trax_atan_unit(ay / ax);
+    // atan of the smaller over the larger magnitude: one division
+    const int swap = ay > ax;
+    const float num = swap ? ax : ay, den = swap ? ay : ax;
+    const float a = trax_atan_unit(num / den);
+    r = swap ? (TRAX_PIO2_F - a) + TRAX_PIO2_LO : a;
     if (xneg) r = (TRAX_PI_F - r) + TRAX_PI_LO;
   }
   return trax_copysign(r, y);
@@ -103,15 +105,13 @@ TRAX_FP float trax_asin_poly(float a) {
 TRAX_FP float trax_acosf(float x) {
   if (x != x) return x;
   if (x > 1.0f || x < -1.0f) return __builtin_nanf("");
-  if (x > 0.5f) {
-    float s = __builtin_sqrtf(0.5f * (1.0f - x));
-    return 2.0f * trax_asin_poly(s);
-  }
-  if (x < -0.5f) {
-    float s = __builtin_sqrtf(0.5f * (1.0f + x));
-    return (TRAX_PI_F - 2.0f * trax_asin_poly(s)) + TRAX_PI_LO;
-  }
-  return (TRAX_PIO2_F - trax_asin_poly(x)) + TRAX_PIO2_LO;
+  // x > 0.5: 2 asin(sqrt((1 - x) / 2)); x < -0.5: pi - 2 asin(sqrt((1 + x) / 2));
+  // else pi/2 - asin(x).  One square root and one polynomial, operands by
+  // select (1 + x == 1 - |x| for negative x, exactly).
+  const int hi = x > 0.5f, lo = x < -0.5f;
+  const float s = __builtin_sqrtf(0.5f * (1.0f - __builtin_fabsf(x)));
+  const float p = trax_asin_poly(hi || lo ? s : x);
+  return hi ? 2.0f * p : lo ? (TRAX_PI_F - 2.0f * p) + TRAX_PI_LO : (TRAX_PIO2_F - p) + TRAX_PIO2_LO;
 }
 
 // OpenCL sign(): 1, -1, the zero itself, or 0 for NaN.
