@@ -107,12 +107,16 @@ typedef struct {
  * TRAX_PREDICT_TILE_SLOTS uint32 (tiles with more slots walk again). */
 #define TRAX_PREDICT_TILE 64
 #define TRAX_PREDICT_TILE_SLOTS 512
+/* The beam-spot pair passes split each outer hit's window into at most this
+ * many runs of z rows (one count per run): two on grids of 128 or more z
+ * rows, one otherwise. */
+#define TRAX_PAIR_PARTS 2
 
 /* Scratch of the count/store phases (device pointers, caller-owned; laid
  * out by trax_workspace_layout). */
 typedef struct {
-  uint32_t *work_offsets;  /* >= max_events * n_triplets + 1 */
-  uint32_t *pair_counts;   /* >= max_hits + 1 */
+  uint32_t *work_offsets;  /* >= max_events * n_triplets + 2 */
+  uint32_t *pair_counts;   /* >= TRAX_PAIR_PARTS * max_hits + 1 */
   uint32_t *cand_counts;   /* >= pair_capacity + 1 */
   uint32_t max_events, max_hits;
   uint64_t pair_capacity;

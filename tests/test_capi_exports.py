@@ -50,7 +50,7 @@ def test_workspace_layout_sizes():
     ws = runtime.WorkspaceC()
     n = lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, 100000, None, ctypes.byref(ws))
     words = (100000 + 63) // 64 + 8
-    need = ((4 * 2 + 1) * 4 + 10001 * 4 + 50001 * 4   # work offsets, pair and candidate counts
+    need = ((4 * 2 + 2) * 4 + 20001 * 4 + 50001 * 4   # work offsets, pair (2 per hit) and candidate counts
             + (50000 + 63) // 64 * 512 * 4            # predictor stash slabs
             + words * 8 + (words + 1) * 4             # filter mask words and their counts
             + 2 * ((50000 + 63) // 64 + 2) * 4)       # candidate slots and triplets per tile of pairs
@@ -65,7 +65,7 @@ def test_workspace_layout_sizes():
     for p in ptrs:
         assert p % 256 == 0 and base <= p < base + n
     # regions in order and disjoint
-    sizes = [(4 * 2 + 1) * 4, 10001 * 4, 50001 * 4, (50000 + 63) // 64 * 512 * 4, words * 8,
+    sizes = [(4 * 2 + 2) * 4, 20001 * 4, 50001 * 4, (50000 + 63) // 64 * 512 * 4, words * 8,
              (words + 1) * 4, ((50000 + 63) // 64 + 2) * 4, ((50000 + 63) // 64 + 2) * 4]
     for (p, sz), q in zip(zip(ptrs, sizes), ptrs[1:] + [base + n]):
         assert p + sz <= q

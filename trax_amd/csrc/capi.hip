@@ -13,6 +13,7 @@
 #include "kernels.h"
 
 static_assert(TRAX_MAX_LAYERS == 16 && TRAX_MAX_TRIPLETS == 8, "table sizes");
+static_assert(TRAX_PAIR_PARTS == trax::kPairParts, "pair parts");
 static_assert(TRAX_PREDICT_TILE == trax::kPredTile && TRAX_PREDICT_TILE_SLOTS == trax::kTileSlots,
               "predictor tile shape");
 
@@ -161,6 +162,12 @@ int make_params(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events, t
   }
   if (cfg->pair_generator != TRAX_PAIRS_BEAMSPOT && cfg->pair_generator != TRAX_PAIRS_SECTOR)
     return fail(ctx, TRAX_ERR_INVALID, "unknown pair generator");
+  // Beam-spot windows span many z rows only on fine z binnings; there two
+  // lanes share each outer hit's window (PU200, 256 z rows: pair count
+  // 148 -> 142 us, store 314 -> 309 us), while on coarse ones (the ttbar
+  // configuration's 50 z rows) the second window evaluation costs more than
+  // the shorter walks save (84 -> 102 us).
+  sp->pair_parts = cfg->n_z >= 128 ? 2u : 1u;
   return TRAX_OK;
 }
 
@@ -344,9 +351,9 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
   const uint64_t words = trax::filter_mask_words(cand_capacity);
   uint64_t off = 0;
   const uint64_t o_work = off;
-  off = align_up(off + ((uint64_t)max_events * cfg->n_triplets + 1) * 4);
+  off = align_up(off + ((uint64_t)max_events * cfg->n_triplets + 2) * 4);
   const uint64_t o_pc = off;
-  off = align_up(off + ((uint64_t)max_hits + 1) * 4);
+  off = align_up(off + ((uint64_t)TRAX_PAIR_PARTS * max_hits + 1) * 4);
   const uint64_t o_cc = off;
   off = align_up(off + (pair_capacity + 1) * 4);
   const uint64_t o_stash = off;
@@ -455,13 +462,14 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
                                              s));
   TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
                                          ws->max_hits, false, ctx->flags, ctx->launch.pairs_block, s));
-  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
-                                            ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s,
+  const uint32_t items_max = sp.pair_parts * ws->max_hits;
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)items_max + 1,
+                                            ws->work_offsets + K + 1, 1, ctx->ctrl, ctx->status, s,
                                             scan_sub(ctx)));
-  // pair_offsets[k] = scanned count at the first outer hit of segment k
-  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->pair_counts, ws->max_hits, ws->work_offsets + K,
+  // pair_offsets[k] = scanned count at the first item of segment k
+  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->pair_counts, items_max, ws->work_offsets + K + 1,
                                                   ws->work_offsets, K + 1, out->pair_offsets,
-                                                  out->totals + 0, s));
+                                                  out->totals + 0, s, sp.pair_parts));
   return TRAX_OK;
 }
 

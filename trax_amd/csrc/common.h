@@ -36,6 +36,8 @@ struct SeedParams {
   // radius: window terms that depend on the layer only, evaluated once on
   // the host with the same FP32 definition the kernels use
   float acos_min_r[TRAX_MAX_LAYERS], acos_max_r[TRAX_MAX_LAYERS];
+  // beam-spot pair work items per outer hit, 1 or 2 (runs of z rows)
+  uint32_t pair_parts;
 };
 
 // Sorted (grid-ordered) hit columns, as left behind by the grid build.

@@ -53,13 +53,17 @@ hipError_t launch_sector_pairs(const SeedParams& sp, const SortedHits& h, const 
                                const uint32_t* work_off, uint32_t* counts, uint2* pairs,
                                uint64_t capacity, uint32_t max_work, bool store, uint32_t* flags,
                                hipStream_t stream);
+// Beam-spot pairs: at most this many work items per outer hit
+// (== TRAX_PAIR_PARTS, checked in the C ABI); the count pass also writes the
+// item count to work_off[K + 1].
+constexpr uint32_t kPairParts = 2;
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                        const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
+                        uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
                         uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
-                                 hipStream_t stream);
+                                 hipStream_t stream, uint32_t mul = 1);
 hipError_t launch_read_last(const uint32_t* arr, const uint32_t* d_index, uint32_t* out,
                             hipStream_t stream);
 hipError_t launch_predict(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,

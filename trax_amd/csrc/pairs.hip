@@ -189,21 +189,33 @@ __device__ __forceinline__ PairItem pair_item(const SeedParams& sp, const Sorted
   return it;
 }
 
-// Two-phase form (count -> prefix sum -> store), used by the plain method.
+// Count and store passes.  Work item w is part w % P of outer hit w / P,
+// P = sp.pair_parts: the z rows of the hit's window split into P consecutive
+// runs, so the items' outputs in item order are the hit's pairs in walk
+// order, and a window of many rows is walked by two lanes (the passes end
+// with their longest walks).
 template <bool STORE>
 __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
                                                const uint32_t* __restrict__ grid,
-                                               const uint32_t* __restrict__ work_off,
+                                               uint32_t* __restrict__ work_off,
                                                uint32_t* __restrict__ counts,  // count: out; store: scanned
                                                uint2* __restrict__ pairs, uint64_t capacity,
                                                uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
-  const uint32_t W = work_off[K];
+  const uint32_t parts = sp.pair_parts, W = work_off[K] * parts;  // items
+  if (!STORE && blockIdx.x == 0 && threadIdx.x == 0) work_off[K + 1] = W;  // the scan's length - 1
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += stride) {
-    const PairItem it = pair_item(sp, h, grid, offs, w);
+    const uint32_t hit = parts == 2u ? w >> 1 : w, part = w - hit * parts;
+    PairItem it = pair_item(sp, h, grid, offs, hit);
+    if (parts == 2u) {
+      const uint32_t lo = it.win.zLowSector, hi = it.win.zHighSector;
+      const uint32_t mid = hi > lo ? lo + (hi - lo) / 2u : lo;
+      it.win.zLowSector = part ? mid : lo;
+      it.win.zHighSector = part ? hi : mid;
+    }
     if (!STORE) {
       uint32_t n = 0;
       pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
@@ -224,17 +236,17 @@ __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
   }
 }
 
-// out[k] = scanned[min(idx[k], last)] for k in [0, n); total -> *d_total.
+// out[k] = scanned[min(idx[k] * mul, last)] for k in [0, n); total -> *d_total.
 __global__ __launch_bounds__(256) void k_gather_offsets(const uint32_t* __restrict__ scanned,
                                                         uint32_t last,
                                                         const uint32_t* __restrict__ last_dev,
                                                         const uint32_t* __restrict__ idx, uint32_t n,
-                                                        uint32_t* __restrict__ out,
+                                                        uint32_t mul, uint32_t* __restrict__ out,
                                                         uint32_t* __restrict__ d_total) {
   uint32_t lim = last;
   if (last_dev) lim = *last_dev < last ? *last_dev : last;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    uint32_t j = idx[k];
+    uint32_t j = idx[k] * mul;
     j = j < lim ? j : lim;
     uint32_t v = scanned[j];
     out[k] = v;
@@ -272,10 +284,10 @@ hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t
 }
 
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                        const uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
+                        uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
                         uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream) {
-  uint32_t blocks = (max_work + block - 1) / block;
+  uint32_t blocks = (uint32_t)(((uint64_t)max_work * sp.pair_parts + block - 1) / block);
   if (blocks == 0) blocks = 1;
   if (blocks > 16384u) blocks = 16384u;
   if (store)
@@ -290,11 +302,11 @@ hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_
 
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,
-                                 hipStream_t stream) {
+                                 hipStream_t stream, uint32_t mul) {
   uint32_t blocks = (n + 255u) / 256u;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_gather_offsets, dim3(blocks), dim3(256), 0, stream, scanned, last, last_dev,
-                     idx, n, out, d_total);
+                     idx, n, mul, out, d_total);
   return hipGetLastError();
 }
 
