@@ -38,7 +38,7 @@ def _both(ctx, sc, batch):
     return s, f, (bf, df)
 
 
-FUSE_BUF, SPILL = 8, 256  # lane buffer depth and spill pool size of the fused walk
+FUSE_BUF, SPILL = 8, 192  # lane buffer depth and spill pool size of the fused walk
 
 
 def _spill_per_tile(per_pair):
