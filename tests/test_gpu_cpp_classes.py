@@ -40,3 +40,30 @@ def test_cpp_classes_match_oracle(tmp_path, config_id, n_events):
     assert np.array_equal(g["tracklet_offsets"], o["tracklet_offsets"])
     assert np.array_equal(rows_sorted(with_segment(g["tracklets"], g["tracklet_offsets"])),
                           rows_sorted(with_segment(o["tracklets"], o["tracklet_offsets"])))
+
+
+@pytest.mark.parametrize("config_id,n_events", [(1, 2), (3, 2)])
+def test_cpp_fused_pass_equals_staged_classes(tmp_path, config_id, n_events):
+    """TripletThetaPhiPredictFilter (trax_seed ... fused) writes the same
+    triplets, in the same order, and the same triplet and candidate offsets
+    as TripletThetaPhiPredictor + TripletThetaPhiFilter, and no candidate
+    records."""
+    from trax_amd import eventfile
+    from trax_amd import events as ev
+    w = ev.config_workload(config_id, n_events)
+    sc, batch = w.seed_config, ev.make_batch(w)
+    fin = str(tmp_path / "events.bin")
+    eventfile.write_events(fin, sc, batch)
+    res = {}
+    for mode in ("staged", "fused"):
+        fout = str(tmp_path / f"{mode}.bin")
+        print(f"trax_seed {mode}: config {config_id}, {n_events} events", flush=True)
+        args = [DRIVER, fin, fout] + (["beamspot", "fused"] if mode == "fused" else [])
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        res[mode] = eventfile.read_results(fout, batch.n_events, len(sc.triplets))
+    s, f = res["staged"], res["fused"]
+    assert len(f["cands"]) == 0 and len(s["cands"]) > 0
+    for k in ("grid", "pairs", "pair_offsets", "cand_offsets", "tracklets", "tracklet_offsets"):
+        assert np.array_equal(s[k], f[k]), k
+    assert len(f["tracklets"]) > 0

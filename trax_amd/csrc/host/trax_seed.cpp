@@ -3,7 +3,12 @@
 // 240-263): GridBuilder, PairGeneratorBeamspot, TripletThetaPhiPredictor,
 // TripletThetaPhiFilter, one event group per input file.
 //
-//   trax_seed <events.bin> <results.bin>
+//   trax_seed <events.bin> <results.bin> [beamspot|sector] [fused]
+//
+// "sector" runs PairGeneratorSector in place of PairGeneratorBeamspot;
+// "fused" runs the predictor and the filter as one pass
+// (TripletThetaPhiPredictFilter): the results then hold no candidate
+// records (count 0), only the candidate offsets.
 //
 // events.bin (little endian): "TRXE", u32 version=1, u32 nEvents, nLayers,
 // nHits, nSectorsZ, nSectorsPhi, nTriplets, f32 minPt, f32 minRadius[nLayers],
@@ -52,8 +57,15 @@ void put_u32(FILE* f, uint32_t v) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 3 && !(argc == 4 && (std::string(argv[3]) == "beamspot" || std::string(argv[3]) == "sector"))) {
-    fprintf(stderr, "usage: %s <events.bin> <results.bin> [beamspot|sector]\n", argv[0]);
+  bool sector = false, fused = false, bad = false;
+  for (int i = 3; i < argc; ++i) {
+    const std::string a = argv[i];
+    if (a == "sector") sector = true;
+    else if (a == "fused") fused = true;
+    else if (a != "beamspot") bad = true;
+  }
+  if (argc < 3 || argc > 5 || bad) {
+    fprintf(stderr, "usage: %s <events.bin> <results.bin> [beamspot|sector] [fused]\n", argv[0]);
     return 2;
   }
   try {
@@ -110,17 +122,25 @@ int main(int argc, char** argv) {
     trax::GridBuilder gridBuilder(ctx, ws);
     gridBuilder.run(hits, 256, es, ls, grid, geom, trip);
     std::unique_ptr<trax::Pairing> pairs;
-    if (argc == 4 && std::string(argv[3]) == "sector") {
+    if (sector) {
       trax::PairGeneratorSector pairGen(ctx, ws, tot);
       pairs = pairGen.run(hits, geom, 256, trip, grid);
     } else {
       trax::PairGeneratorBeamspot pairGen(ctx, ws, tot);
       pairs = pairGen.run(hits, geom, 256, trip, grid);
     }
-    trax::TripletThetaPhiPredictor predictor(ctx, ws, tot);
-    auto cands = predictor.run(hits, geom, 256, trip, grid, *pairs);
-    trax::TripletThetaPhiFilter filter(ctx, ws, tot);
-    auto trk = filter.run(hits, grid, *pairs, *cands, 256, trip, geom);
+    std::unique_ptr<trax::Pairing> cands;
+    std::unique_ptr<trax::TrackletCollection> trk;
+    if (fused) {  // no candidate records: an empty candidate section with its offsets
+      cands = std::make_unique<trax::Pairing>(0, E, P);
+      trax::TripletThetaPhiPredictFilter predictFilter(ctx, ws, tot);
+      trk = predictFilter.run(hits, geom, 256, trip, grid, *pairs, &cands->pairingOffsets);
+    } else {
+      trax::TripletThetaPhiPredictor predictor(ctx, ws, tot);
+      cands = predictor.run(hits, geom, 256, trip, grid, *pairs);
+      trax::TripletThetaPhiFilter filter(ctx, ws, tot);
+      trk = filter.run(hits, grid, *pairs, *cands, 256, trip, geom);
+    }
     ctx.finish();
     if (uint32_t f = ctx.flags()) throw trax::Error("device status flags " + std::to_string(f));
 

@@ -465,6 +465,57 @@ class TripletThetaPhiFilter : public KernelWrapper {
   Totals& tot;
 };
 
+// The predictor and the filter as one pass (trax_triplet_predict_filter; no
+// class of the reference corresponds): for callers that use the triplets
+// only.  The triplet count is known only after the pass, so the pass runs
+// once with a first guess of the capacity and again when that overflowed.
+// `candOffsets` receives the candidate offsets the predictor would have
+// written.
+class TripletThetaPhiPredictFilter : public KernelWrapper {
+ public:
+  TripletThetaPhiPredictFilter(Context& c, Workspace& w, Totals& t) : KernelWrapper(c), ws(w), tot(t) {}
+  std::unique_ptr<TrackletCollection> run(HitCollection& hits, const GeometrySupplement& geom,
+                                          int /*nThreads*/, const TripletConfigurations& t, const Grid& grid,
+                                          const Pairing& pairs, DeviceArray<uint32_t>* candOffsets = nullptr) {
+    const trax_seed_config cfg = make_config(grid.config, geom, t);
+    const uint32_t E = grid.config.nEvents, P = (uint32_t)t.size();
+    ws.ensure(cfg, E, (uint32_t)hits.size(), pairs.count);
+    trax_hits s = hits.sorted();
+    DeviceArray<uint32_t> own((size_t)E * P + 1);
+    DeviceArray<uint32_t>& co = candOffsets ? *candOffsets : own;
+    if (co.size() < (size_t)E * P + 1) co.resize((size_t)E * P + 1);
+    uint64_t cap = pairs.count / 16 + 1024;
+    // an earlier stage's status bits would be lost to the overflow check below
+    if (const uint32_t f = ctx.flags(true)) throw Error("device status flags " + std::to_string(f));
+    for (;;) {
+      auto trk = std::make_unique<TrackletCollection>(cap, E, P);
+      trax_seed_outputs o{};
+      o.pairs = pairs.pairing.get();
+      o.pair_capacity = pairs.count;
+      o.pair_offsets = pairs.pairingOffsets.get();
+      o.cand_offsets = co.get();
+      o.t1 = trk->hit1.get(), o.t2 = trk->hit2.get(), o.t3 = trk->hit3.get();
+      o.tracklet_capacity = cap;
+      o.tracklet_offsets = trk->trackletOffsets.get();
+      o.totals = tot.d.get();
+      ctx.check(trax_triplet_predict_filter(ctx.ctx(), &cfg, E, &s, grid.boundary.get(), ws.get(), &o,
+                                            ctx.stream()),
+                "predict + filter");
+      const uint32_t n = tot.read(ctx, 2);
+      const uint32_t f = ctx.flags(true);
+      if (!(f & TRAX_FLAG_TRACKLET_OVERFLOW) && n <= cap) {
+        trk->count = n;
+        return trk;
+      }
+      cap = n;
+    }
+  }
+
+ private:
+  Workspace& ws;
+  Totals& tot;
+};
+
 // TripletConnectivity (TripletConnectivity.h:27-56, .cpp:21-71): adds to
 // following.connectivity, per triplet, the connectivity + 1 of every base
 // triplet whose outer hit is its inner hit (backward: whose inner hit is its
