@@ -217,6 +217,28 @@ __device__ __forceinline__ TileScan tile_scan_lookback(uint32_t v, uint32_t tile
 }
 
 // Makes a wave's LDS writes visible to the other lanes of the same wave.
+// Blocks go to the eight XCDs round-robin (block b on XCD b % 8), each XCD
+// with its own L2.  xcd_share splits n work items into eight contiguous
+// shares, share x for the blocks of XCD x, which stride through it: the
+// items a round of an XCD's waves takes are neighbours (one event's hits and
+// grid cells stay in that XCD's L2), and the shares stay balanced however
+// far the capacity-sized grid exceeds the work.  `unit` is the block's
+// first item of a round (items per block and round: `per_block`).  For a
+// grid that is not a multiple of 8 it degenerates to the plain grid stride.
+struct XcdShare {
+  uint32_t begin, end, stride;
+};
+__device__ __forceinline__ XcdShare xcd_share(uint32_t n, uint32_t per_block, uint32_t local) {
+  const uint32_t nb = gridDim.x;
+  if (nb & 7u) return XcdShare{blockIdx.x * per_block + local, n, nb * per_block};
+  const uint32_t x = blockIdx.x & 7u;
+  const uint32_t lo = (uint32_t)((uint64_t)n * x / 8u), hi = (uint32_t)((uint64_t)n * (x + 1u) / 8u);
+  return XcdShare{lo + (blockIdx.x >> 3) * per_block + local, hi, (nb >> 3) * per_block};
+}
+
+// Block count rounded up to a multiple of 8 (see xcd_share).
+inline uint32_t xcd_grid(uint32_t blocks) { return blocks > 8u ? (blocks + 7u) & ~7u : blocks; }
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();

@@ -206,8 +206,8 @@ __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
   const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
   const uint32_t parts = sp.pair_parts, W = work_off[K] * parts;  // items
   if (!STORE && blockIdx.x == 0 && threadIdx.x == 0) work_off[K + 1] = W;  // the scan's length - 1
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += stride) {
+  const XcdShare sh = xcd_share(W, blockDim.x, threadIdx.x);
+  for (uint32_t w = sh.begin; w < sh.end; w += sh.stride) {
     const uint32_t hit = parts == 2u ? w >> 1 : w, part = w - hit * parts;
     PairItem it = pair_item(sp, h, grid, offs, hit);
     if (parts == 2u) {
@@ -290,6 +290,7 @@ hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_
   uint32_t blocks = (uint32_t)(((uint64_t)max_work * sp.pair_parts + block - 1) / block);
   if (blocks == 0) blocks = 1;
   if (blocks > 16384u) blocks = 16384u;
+  blocks = xcd_grid(blocks);
   if (store)
     hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
                        counts, pairs, capacity, flags);

@@ -696,8 +696,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
   const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
-  const uint32_t nwaves = gridDim.x * 4u;
-  for (uint32_t t = blockIdx.x * 4u + wave; t < ntiles; t += nwaves) {
+  const XcdShare sh = xcd_share(ntiles, 4u, wave);
+  for (uint32_t t = sh.begin; t < sh.end; t += sh.stride) {
     if (!fused_tile(sp, h, grid, pairs, offs, nP, t, s_lds[wave], counts, tile_counts, acc_slab, acc_tiles) &&
         lane_id() == 0)
       acc_tiles[t] = kTileRedo;  // left to k_acc_redo
@@ -881,7 +881,7 @@ hipError_t launch_predict_filter_walk(const SeedParams& sp, const SortedHits& h,
                                       const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                                       uint64_t pair_capacity, uint32_t* counts, uint32_t* tile_counts,
                                       uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream) {
-  hipLaunchKernelGGL(k_predict_filter_walk, dim3(tile_blocks(pair_capacity)), dim3(256), offsets_lds_bytes(sp),
+  hipLaunchKernelGGL(k_predict_filter_walk, dim3(xcd_grid(tile_blocks(pair_capacity))), dim3(256), offsets_lds_bytes(sp),
                      stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, tile_counts, acc_slab,
                      acc_tiles);
   return hipGetLastError();
