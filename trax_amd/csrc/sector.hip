@@ -105,8 +105,8 @@ __global__ __launch_bounds__(256) void k_sector_pairs(SeedParams sp, SortedHits 
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
   const uint32_t W = work_off[K];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += stride) {
+  const XcdShare sh = xcd_share(W, blockDim.x, threadIdx.x);
+  for (uint32_t w = sh.begin; w < sh.end; w += sh.stride) {
     const SectorItem it = sector_item(sp, h, grid, offs, w);
     if (!STORE) {
       uint32_t n = 0;
@@ -135,6 +135,7 @@ hipError_t launch_sector_pairs(const SeedParams& sp, const SortedHits& h, const 
   uint32_t blocks = (max_work + 255u) / 256u;
   if (blocks == 0) blocks = 1;
   if (blocks > 16384u) blocks = 16384u;
+  blocks = xcd_grid(blocks);
   if (store)
     hipLaunchKernelGGL(k_sector_pairs<true>, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
                        grid, work_off, counts, pairs, capacity, flags);
