@@ -129,10 +129,26 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
-def cpu_baseline(sc, batch, threads: int, reps: int = 3, single_sample: int = 8):
+def measured_issue(workload: str, kernel: str):
+    """VALU issue of `kernel` from the committed rocprofv3 SQ counter
+    summary of this workload (SQ_ACTIVE_INST_VALU over the SIMDs' issue
+    slots, one wave64 VALU instruction per SIMD every 4 cycles), or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_issue_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
+            if parts.get("kernel") == kernel and "valu_issue_frac" in parts:
+                return {k: float(v) for k, v in parts.items() if k != "kernel"}
+    return None
+
+
+def cpu_baseline(sc, batch, threads: int, reps: int = 5, single_sample: int = 16):
     """The oracle chain over every event of the batch, one event per task on a
     pool of `threads` (ctypes releases the GIL): the median of `reps` runs in
-    events/s; plus one thread over the first `single_sample` events."""
+    events/s; plus one thread over the first `single_sample` events.  Also
+    returns the last run's per-event results (the checker of the GPU run)."""
     import concurrent.futures as cf
     import statistics
 
@@ -140,20 +156,60 @@ def cpu_baseline(sc, batch, threads: int, reps: int = 3, single_sample: int = 8)
     from trax_amd import events as ev
 
     singles = [ev.concat_events([batch], subset=[e]) for e in range(batch.n_events)]
-    runs, found = [], 0
+    runs, res = [], []
     for _ in range(reps):
         t0 = time.perf_counter()
         with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-            res = list(ex.map(lambda b: len(bridge.run_chain(sc, b)["tracklets"]), singles))
+            res = list(ex.map(lambda b: bridge.run_chain(sc, b), singles))
         runs.append(time.perf_counter() - t0)
-        found = sum(res)
     m = min(single_sample, len(singles))
     t0 = time.perf_counter()
     for b in singles[:m]:
         bridge.run_chain(sc, b)
     one = m / (time.perf_counter() - t0)
     dt = statistics.median(runs)
-    return batch.n_events / dt, found, dt, one, m
+    return batch.n_events / dt, res, dt, one, m, runs
+
+
+def limiter(kernel: str, issue):
+    """What bounds the dominant kernel, in words, from its counters."""
+    if kernel in ("predict_filter", "predict_walk", "filter_mask", "pairs") :
+        txt = ("instruction issue and gather latency, not HBM: per-lane window walks "
+               "(dependent (z, phi) gathers, then compares) and the cuts' FP32 arithmetic")
+        if issue:
+            txt += f"; VALU issue slots used: {100 * issue['valu_issue_frac']:.0f}%"
+        return txt
+    return "HBM"
+
+
+def check_against_oracle(got, batch, P: int, per_event) -> int:
+    """Every event's triplets (as event-relative input-hit indices, per layer
+    triplet) and segment sizes in the GPU batch result equal the oracle's on
+    that event alone; the batch offsets are the running sums.  Raises on any
+    difference; returns the number of triplets compared."""
+    import numpy as np
+
+    sizes = np.concatenate([np.diff(o["tracklet_offsets"].astype(np.int64)) for o in per_event])
+    want_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    if not np.array_equal(got["tracklet_offsets"], want_off):
+        raise SystemExit("triplet offsets differ from the oracle")
+
+    def keys(rows, lt):
+        r = rows.astype(np.uint64)
+        return np.sort((lt.astype(np.uint64) << np.uint64(60)) | (r[:, 0] << np.uint64(40)) |
+                       (r[:, 1] << np.uint64(20)) | r[:, 2])
+    off, oi = got["tracklet_offsets"].astype(np.int64), got["orig_index"].astype(np.int64)
+    for e, o in enumerate(per_event):
+        base = int(batch.event_offset[e])
+        seg = off[e * P:(e + 1) * P + 1]
+        g_rows = oi[got["tracklets"][seg[0]:seg[-1]]] - base
+        g_lt = np.repeat(np.arange(P), np.diff(seg))
+        o_off = o["tracklet_offsets"].astype(np.int64)
+        o_rows = o["orig_index"].astype(np.int64)[o["tracklets"]]
+        o_lt = np.repeat(np.arange(P), np.diff(o_off))
+        if not np.array_equal(keys(g_rows, g_lt), keys(o_rows, o_lt)):
+            raise SystemExit(f"event {e}: triplet set differs from the oracle")
+    return int(off[-1])
 
 
 def real_events(args):
@@ -355,16 +411,17 @@ def main():
     grid_gbs = kern["grid_build"]["gbs"]
     grid_stage_gbs = grid_stage_bytes(counts) / (kern["grid_build"]["ms"] * 1e-3) / 1e9
     log(f"grid build {grid_gbs:.0f} GB/s, scan(2^26) {scan_gbs:.0f} GB/s, dominant {dominant}")
-    copy = None
-    if args.profile:
-        # elementwise kernel of known traffic (256 MiB read, 256 MiB written):
-        # the calibration kernel for FETCH_SIZE / WRITE_SIZE
-        src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        copy_ms = time_events(lambda: torch.neg(src, out=dst), 5, torch)
-        copy = {"bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4, "ms": copy_ms,
-                "gbs": 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9}
-        log(f"copy 256 MiB: {copy_ms:.3f} ms, {copy['gbs']:.0f} GB/s")
+    # elementwise kernel of known traffic (256 MiB read, 256 MiB written): the
+    # measured copy ceiling of this device, and the calibration kernel of the
+    # FETCH_SIZE / WRITE_SIZE counter runs
+    src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    copy_ms = time_events(lambda: torch.neg(src, out=dst), 10, torch)
+    copy = {"kernel": "torch.neg on 2^26 int32", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4,
+            "ms": copy_ms, "gbs": 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9}
+    copy["frac_of_peak"] = copy["gbs"] / HBM_PEAK_GBS
+    log(f"copy 256 MiB: {copy_ms:.3f} ms, {copy['gbs']:.0f} GB/s")
+    del src, dst
 
     # ---- PCIe: what a caller starting from host buffers adds (pinned copies of
     # the input columns and layout up, the triplet columns and offsets down)
@@ -386,7 +443,25 @@ def main():
     log(f"pcie: up {h2d_ms:.3f} ms, down {d2h_ms:.3f} ms, inclusive {pcie['events_per_sec']:.0f} events/s")
 
     # ---- plain path: one event per launch sequence, host reads between stages
-    plain = None
+    plain = plain_c1 = None
+    if not args.no_plain and not args.event_file:
+        # the plain method on its own workload: pile-up-50 events, three
+        # barrel layers, one event per launch sequence
+        w1 = ev.config_workload(1, 64)
+        b1 = ev.make_batch(w1)
+        s1 = [runtime.DeviceBatch.upload(ev.concat_events([b1], subset=[e])) for e in range(b1.n_events)]
+        p1, c1, t1 = runtime.default_capacities(max(d.n_hits for d in s1))
+        pb1 = runtime.ChainBuffers(w1.seed_config, 1, max(d.n_hits for d in s1), p1, c1, t1)
+        runtime.run_plain_event(ctx, pb1, s1[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n1 = sum(runtime.run_plain_event(ctx, pb1, d) for d in s1)
+        torch.cuda.synchronize()
+        dt1 = time.perf_counter() - t0
+        plain_c1 = {"workload": w1.name, "events": len(s1), "hits": b1.n_hits, "triplets": n1,
+                    "events_per_sec": len(s1) / dt1, "ms_per_event": dt1 * 1e3 / len(s1)}
+        del pb1, s1
+        log(f"plain ({w1.name}): {plain_c1['ms_per_event']:.3f} ms/event, {plain_c1['events_per_sec']:.0f} events/s")
     if not args.no_plain:
         singles = [runtime.DeviceBatch.upload(ev.concat_events([batch], subset=[e]))
                    for e in range(batch.n_events)]
@@ -412,17 +487,22 @@ def main():
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         threads = max(1, min(args.cpu_threads, avail))
         log(f"cpu baseline on {threads} threads ({avail} available)")
-        cps, ctrip, cdt, one, m = cpu_baseline(sc, batch, threads)
-        if ctrip != n_trk:
-            raise SystemExit(f"CPU oracle found {ctrip} triplets, GPU {n_trk}")
+        cps, per_event, cdt, one, m, runs = cpu_baseline(sc, batch, threads)
+        # the timed GPU step's triplets (the replayed graph's output) against
+        # the oracle, event by event and segment by segment
+        checked = check_against_oracle(got, batch, len(sc.triplets), per_event)
+        log(f"oracle check: {checked} triplets of {batch.n_events} events equal")
         cpu = {"value": cps, "unit": "events/s", "cores": threads, "kind": "port",
                "sample": f"{batch.n_events} events of the same workload, C oracle chain, one event per task "
-                         f"on {threads} threads, median of 3 runs ({cdt:.2f} s wall each)",
+                         f"on {threads} threads, median of {len(runs)} runs ({cdt:.2f} s wall, runs "
+                         f"{', '.join(f'{r:.2f}' for r in runs)} s)",
                "single_thread": {"value": one, "unit": "events/s", "cores": 1,
-                                 "sample": f"the first {m} events on one thread"}}
+                                 "sample": f"the first {m} events on one thread"},
+               "triplets_checked_vs_gpu": checked}
 
     dom = kern[dominant]
     traffic = measured_traffic(w.name, dominant)
+    issue = measured_issue(w.name, dominant)
     out = {
         "metric": "events/sec (batched seeding chain, 1 GPU)",
         "value": ev_s,
@@ -444,6 +524,7 @@ def main():
         "counts": {"pairs": n_pairs, "candidates": n_cands, "triplets": n_trk},
         "plain": plain,
         "batched_over_plain": (ev_s / plain["events_per_sec"]) if plain else None,
+        "plain_config1": plain_c1,
         "pcie_inclusive": pcie,
         "stages_ms": st,
         "kernels": kern,
@@ -458,7 +539,13 @@ def main():
                           "frac": scan_gbs / HBM_PEAK_GBS, "ms": scan_ms},
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "ms": dom["ms"], "algorithmic_bytes": dom["bytes"]},
+                     "traffic_source": f"profiles/hbm_traffic_{w.name}.txt (FETCH_SIZE + WRITE_SIZE runs of "
+                                       f"bench.py --profile, copy-kernel calibrated)" if traffic else None,
+                     "traffic_gbs": traffic / (dom["ms"] * 1e-3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+                     "ms": dom["ms"], "algorithmic_bytes": dom["bytes"],
+                     "limiter": limiter(dominant, issue),
+                     "valu_issue": issue},
         "cpu_baseline": cpu,
         "connectivity": conn,
         "copy_calibration": copy,

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TRAX_ABI_VERSION 6
+#define TRAX_ABI_VERSION 7
 
 #define TRAX_OK 0
 #define TRAX_ERR_INVALID (-1)  /* bad argument or configuration */
@@ -113,10 +113,12 @@ typedef struct {
 #define TRAX_PAIR_PARTS 2
 
 /* Scratch of the count/store phases (device pointers, caller-owned; laid
- * out by trax_workspace_layout). */
+ * out by trax_workspace_layout).  Every stage call checks the configuration
+ * against the sizes recorded here (n_triplets, pair_items, max_events, and
+ * for trax_seed_chain max_hits) and rejects one the layout cannot hold. */
 typedef struct {
   uint32_t *work_offsets;  /* >= max_events * n_triplets + 2 */
-  uint32_t *pair_counts;   /* >= TRAX_PAIR_PARTS * max_hits + 1 */
+  uint32_t *pair_counts;   /* >= pair_items + 1 */
   uint32_t *cand_counts;   /* >= pair_capacity + 1 */
   uint32_t max_events, max_hits;
   uint64_t pair_capacity;
@@ -128,6 +130,11 @@ typedef struct {
                                       slots per tile of pairs, then their scan, then the tile count */
   uint32_t *trk_tile_counts;       /* >= ceil(pair_capacity / TRAX_PREDICT_TILE) + 2: accepted
                                       triplets per tile of pairs (trax_triplet_predict_filter) */
+  uint32_t n_triplets; /* layer triplets the layout was sized for (work_offsets) */
+  /* pair work items pair_counts holds: TRAX_PAIR_PARTS * max_hits times the
+   * largest number of layer triplets sharing one middle layer (each such
+   * triplet walks the middle layer's hits once) */
+  uint32_t pair_items;
 } trax_workspace;
 
 /* Output buffers of the pair, prediction and filter stages.  Capacities are
@@ -210,6 +217,18 @@ int trax_prefix_sum(trax_ctx *ctx, uint32_t *d_data, uint64_t n, void *stream);
  * n_add, n_max) (d_n may be NULL: n = n_max).  Graph-capture safe. */
 int trax_prefix_sum_dev(trax_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint64_t n_max,
                         const uint32_t *d_n, uint32_t n_add, void *stream);
+
+/* Segmented in-place exclusive scan: the scan restarts at 0 at the start of
+ * every segment.  d_seg_offsets (device, n_segments + 1 entries,
+ * non-decreasing, first 0, last n) gives segment k = [off[k], off[k+1]),
+ * e.g. one segment per event as the reference's per-event offset tables
+ * (EventSupplement / LayerSupplement) lay them out; the reference scans
+ * such tables segment by segment with PrefixSum::run(cl_mem, uint size,
+ * uint nThreads, std::vector<cl_event>&), PrefixSum.h:42-43, one call per
+ * range.  At most 2^22 segments (the context's look-back words hold the
+ * segment bases).  No host read; graph-capture safe. */
+int trax_prefix_sum_segmented(trax_ctx *ctx, uint32_t *d_data, uint64_t n, const uint32_t *d_seg_offsets,
+                              uint32_t n_segments, void *stream);
 
 /* ------------------------------------------------------------ grid build */
 /* GridBuilder::run(HitCollection&, nThreads, EventSupplement, LayerSupplement,
@@ -331,6 +350,29 @@ int trax_triplet_predict_filter(trax_ctx *ctx, const trax_seed_config *cfg, uint
 int trax_seed_chain(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
                     const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,
                     const trax_workspace *ws, trax_seed_outputs *out, void *stream);
+
+/* The whole chain as one HIP graph, for callers that run the same batch
+ * shape again and again: trax_chain_graph_capture records trax_seed_chain
+ * with these exact arguments (buffers, capacities, configuration) into a
+ * graph and instantiates it; trax_chain_graph_launch replays it on `stream`
+ * (one graph launch instead of ~14 kernel launches, no host work per
+ * stage).  It replaces the reference's stage sequence with its three
+ * blocking host reads (PairGeneratorBeamspot.cpp:120,
+ * TripletThetaPhiPredictor.cpp:59, TripletThetaPhiFilter.cpp:82; called
+ * from TripletBuilder.cpp:240-263).  The buffers named at capture are read
+ * and written by every launch: keep them alive, and change their contents
+ * (new hits of the same event layout sizes) between launches as needed; the
+ * layout arrays are read on the device, so event sizes may change as long as
+ * n_events and n_hits stay within what the buffers hold (n_events and n_hits
+ * themselves are part of the recording).  Status bits land in the context
+ * (trax_ctx_flags).  Capture runs on a private stream of the context's
+ * device, so `stream` of the launch may be any stream of that device. */
+typedef struct trax_graph trax_graph;
+int trax_chain_graph_capture(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits *in,
+                             const trax_layout *layout, trax_hits *sorted, uint32_t *d_grid,
+                             const trax_workspace *ws, trax_seed_outputs *out, trax_graph **graph_out);
+int trax_chain_graph_launch(trax_graph *graph, void *stream);
+int trax_chain_graph_destroy(trax_graph *graph);
 
 /* ----------------------------------------------------------- connectivity */
 /* A triplet set on the device: the hit and connectivity columns of the

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_only_entry_points():
     lib = runtime.lib()
-    assert lib.trax_abi_version() == 6
+    assert lib.trax_abi_version() == 7
     # connectivity table: a power of two >= 2 * n_base slots (at least 64), plus one, 16 B each
     for n, slots in ((0, 64), (1, 64), (32, 64), (33, 128), (1000, 2048), (1 << 20, 1 << 21)):
         assert lib.trax_connectivity_workspace_bytes(n) == (slots + 1) * 16
@@ -57,6 +57,8 @@ def test_workspace_layout_sizes():
     assert need <= n <= need + 8 * 256
     assert ws.max_events == 4 and ws.max_hits == 10000 and ws.pair_capacity == 50000
     assert ws.cand_capacity == 100000
+    # the layout records what it was sized for: the stage calls check against it
+    assert ws.n_triplets == 2 and ws.pair_items == 2 * 10000
     base = 1 << 20
     lib.trax_workspace_layout(ctypes.byref(cfg), 4, 10000, 50000, 100000, ctypes.c_void_p(base),
                               ctypes.byref(ws))
@@ -84,3 +86,21 @@ def test_context_requires_a_device():
         pytest.skip("a device is present")
     with pytest.raises(runtime.TraxError):
         runtime.Context(0)
+
+
+def test_workspace_pair_items_cover_shared_middle_layers():
+    """Layer triplets that share a middle layer each walk its hits once in
+    the beam-spot pair passes: the pair counts are sized for the largest
+    such group (triplets (1,3,4), (2,3,5), (3,4,5): middle layer 3 twice)."""
+    lib = runtime.lib()
+    sc = ev.config_workload(3, 1).seed_config
+    sc.triplets = [ev.LayerTriplet(1, 3, 4), ev.LayerTriplet(2, 3, 5), ev.LayerTriplet(3, 4, 5)]
+    cfg = runtime.seed_config_c(sc)
+    ws = runtime.WorkspaceC()
+    n = lib.trax_workspace_layout(ctypes.byref(cfg), 2, 5000, 1000, 1000, None, ctypes.byref(ws))
+    assert n > 0 and ws.n_triplets == 3 and ws.pair_items == 2 * 2 * 5000
+    # distinct middle layers: one walk per hit
+    sc.triplets = [ev.LayerTriplet(1, 2, 3), ev.LayerTriplet(2, 3, 4), ev.LayerTriplet(3, 4, 5)]
+    cfg = runtime.seed_config_c(sc)
+    lib.trax_workspace_layout(ctypes.byref(cfg), 2, 5000, 1000, 1000, None, ctypes.byref(ws))
+    assert ws.pair_items == 2 * 5000

@@ -343,3 +343,66 @@ def test_pu200_results_do_not_depend_on_z_binning():
     a, b = run(64), run(256)
     assert a[0] == b[0] and a[1] == b[1]
     assert a[2] == b[2] and a[3] == b[3] and len(a[3]) > 1000
+
+
+# ------------------------------------------- batch = events alone, segments
+@pytest.mark.parametrize("config_id,n_events", [(2, 6), (3, 2)])
+def test_batch_equals_events_alone(config_id, n_events):
+    """The oracle on an event group gives, per event, what it gives on each
+    event alone (every (event, layer) segment is processed on its own), so
+    a batch can be checked against the oracle event by event."""
+    from helpers import compare_batch_per_event, oracle_per_event
+    w = ev.config_workload(config_id, n_events)
+    b = ev.make_batch(w)
+    whole = bridge.run_chain(w.seed_config, b)
+    compare_batch_per_event(whole, b, len(w.seed_config.triplets), oracle_per_event(w.seed_config, b))
+
+
+@pytest.mark.parametrize("config_id,n_events", [(3, 2), (2, 4)])
+def test_segment_rows_belong_to_their_segment(config_id, n_events):
+    """Segment k = (event e, layer triplet p) of the pairs, the candidates and
+    the triplets holds exactly the rows whose first hit lies in event e on
+    layer p + 1 (and whose other hits lie in e on the triplet's layers):
+    every row of [off[k], off[k+1]) satisfies that, and off[k+1] - off[k]
+    equals the number of such rows in the whole array."""
+    w = ev.config_workload(config_id, n_events)
+    sc = w.seed_config
+    b = ev.make_batch(w)
+    o = bridge.run_chain(sc, b)
+    P, E = len(sc.triplets), b.n_events
+    lay, evt = o["layer"].astype(np.int64), o["evt"].astype(np.int64)
+    l2 = np.array([t.layer2 for t in sc.triplets])
+    l3 = np.array([t.layer3 for t in sc.triplets])
+    cands = o["cands"]
+    written = cands[:, 0] != UNWRITTEN
+    cand_rows = np.zeros((len(cands), 3), np.int64)
+    cand_rows[written, :2] = o["pairs"][cands[written, 0]]
+    cand_rows[written, 2] = cands[written, 1]
+    stages = {"pairs": (o["pairs"].astype(np.int64), o["pair_offsets"], np.ones(len(o["pairs"]), bool)),
+              "cands": (cand_rows, o["cand_offsets"], written),
+              "tracklets": (o["tracklets"].astype(np.int64), o["tracklet_offsets"],
+                            np.ones(len(o["tracklets"]), bool))}
+    for name, (rows, off, ok) in stages.items():
+        assert off[0] == 0 and off[-1] == len(rows) and np.all(np.diff(off.astype(np.int64)) >= 0), name
+        seg = np.searchsorted(off, np.arange(len(rows)), side="right") - 1
+        e_of, p_of = seg // P, seg % P
+        r, e_r, p_r = rows[ok], e_of[ok], p_of[ok]
+        assert np.all(evt[r[:, 0]] == e_r) and np.all(lay[r[:, 0]] == p_r + 1), name
+        assert np.all(evt[r[:, 1]] == e_r) and np.all(lay[r[:, 1]] == l2[p_r]), name
+        if r.shape[1] > 2:
+            assert np.all(evt[r[:, 2]] == e_r) and np.all(lay[r[:, 2]] == l3[p_r]), name
+        # every segment's size equals the number of rows that belong to it
+        # (unwritten candidate slots stay with the pair that owns them)
+        owner = rows[:, 0] if name != "cands" else np.where(written, cand_rows[:, 0], -1)
+        for k in range(E * P):
+            e, p = divmod(k, P)
+            if name == "cands":
+                # a segment's slots are the budgets of its own pairs
+                pk = slice(int(o["pair_offsets"][k]), int(o["pair_offsets"][k + 1]))
+                own = np.isin(cands[:, 0], np.arange(pk.start, pk.stop)) | \
+                    ((~written) & (seg == k))
+                assert own.sum() == off[k + 1] - off[k], (name, k)
+            else:
+                belongs = (evt[owner] == e) & (lay[owner] == p + 1)
+                assert belongs.sum() == off[k + 1] - off[k], (name, k)
+        assert len(r) > 0, name

@@ -192,14 +192,53 @@ int check_ctx(trax_ctx* ctx) {
   return TRAX_OK;
 }
 
-int check_ws(trax_ctx* ctx, const trax_workspace* ws, uint32_t n_events, uint32_t n_triplets) {
+// Largest number of layer triplets that share one middle layer: the
+// beam-spot pair passes walk a middle layer's hits once per such triplet.
+uint32_t layer2_multiplicity(const trax_seed_config* cfg) {
+  uint32_t m = 1;
+  for (uint32_t p = 0; p < cfg->n_triplets && p < TRAX_MAX_TRIPLETS; ++p) {
+    uint32_t c = 0;
+    for (uint32_t q = 0; q < cfg->n_triplets && q < TRAX_MAX_TRIPLETS; ++q)
+      c += cfg->triplets[q].layer2 == cfg->triplets[p].layer2 ? 1u : 0u;
+    m = c > m ? c : m;
+  }
+  return m;
+}
+
+// Pair work items a batch of max_hits hits can need under cfg.
+uint64_t pair_items_needed(const trax_seed_config* cfg, uint32_t max_hits) {
+  return (uint64_t)TRAX_PAIR_PARTS * max_hits * layer2_multiplicity(cfg);
+}
+
+// The workspace must have been laid out for (at least) this configuration:
+// its work offsets for n_triplets per event, its pair counts for the pair
+// work items, its event count.
+int check_ws(trax_ctx* ctx, const trax_workspace* ws, uint32_t n_events, const trax_seed_config* cfg) {
   if (!ws || !ws->work_offsets || !ws->pair_counts || !ws->cand_counts)
     return fail(ctx, TRAX_ERR_INVALID, "incomplete workspace");
   if (n_events > ws->max_events) return fail(ctx, TRAX_ERR_INVALID, "more events than the workspace holds");
+  if (cfg->n_triplets > ws->n_triplets)
+    return fail(ctx, TRAX_ERR_INVALID, "workspace laid out for fewer layer triplets than the configuration has");
+  if (pair_items_needed(cfg, ws->max_hits) > ws->pair_items)
+    return fail(ctx, TRAX_ERR_INVALID,
+                "workspace pair counts too small for this configuration (layer triplets sharing a middle layer)");
   if ((((uintptr_t)ws->pair_counts) | ((uintptr_t)ws->cand_counts)) & 15u)
     return fail(ctx, TRAX_ERR_INVALID, "workspace counts must be 16-byte aligned");
-  (void)n_triplets;
   return TRAX_OK;
+}
+
+// Pair work items of the beam-spot passes: one or two per middle-layer hit
+// and layer triplet walking that layer (validated against the workspace by
+// check_ws).  The count kernel clamps larger work lists to this and raises
+// TRAX_FLAG_LAYOUT (hits beyond max_hits).
+uint32_t pair_items_max(const trax::SeedParams& sp, const trax_seed_config* cfg, const trax_workspace* ws) {
+  const uint64_t need = (uint64_t)sp.pair_parts * ws->max_hits * layer2_multiplicity(cfg);
+  return need < ws->pair_items ? (uint32_t)need : ws->pair_items;
+}
+
+// Sector pairs: one work item per inner-layer hit (inner layers are distinct).
+uint32_t sector_items_max(const trax_workspace* ws) {
+  return ws->max_hits < ws->pair_items ? ws->max_hits : ws->pair_items;
 }
 
 int scan_sub(const trax_ctx* ctx) { return (int)(ctx->launch.scan_tile / 1024u); }
@@ -347,13 +386,15 @@ extern "C" float trax_min_radius_curvature(float min_pt) {
 extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t max_events,
                                           uint32_t max_hits, uint64_t pair_capacity,
                                           uint64_t cand_capacity, void* base, trax_workspace* ws) {
-  if (!cfg) return 0;
+  if (!cfg || cfg->n_triplets > TRAX_MAX_TRIPLETS) return 0;
+  const uint64_t items = pair_items_needed(cfg, max_hits);
+  if (items >= 0xFFFFFFFFull) return 0;
   const uint64_t words = trax::filter_mask_words(cand_capacity);
   uint64_t off = 0;
   const uint64_t o_work = off;
   off = align_up(off + ((uint64_t)max_events * cfg->n_triplets + 2) * 4);
   const uint64_t o_pc = off;
-  off = align_up(off + ((uint64_t)TRAX_PAIR_PARTS * max_hits + 1) * 4);
+  off = align_up(off + (items + 1) * 4);
   const uint64_t o_cc = off;
   off = align_up(off + (pair_capacity + 1) * 4);
   const uint64_t o_stash = off;
@@ -381,6 +422,8 @@ extern "C" uint64_t trax_workspace_layout(const trax_seed_config* cfg, uint32_t 
     ws->filter_counts = b ? reinterpret_cast<uint32_t*>(b + o_fc) : nullptr;
     ws->cand_tile_counts = b ? reinterpret_cast<uint32_t*>(b + o_tc) : nullptr;
     ws->trk_tile_counts = b ? reinterpret_cast<uint32_t*>(b + o_ac) : nullptr;
+    ws->n_triplets = cfg->n_triplets;
+    ws->pair_items = (uint32_t)items;
   }
   return off;
 }
@@ -404,6 +447,24 @@ extern "C" int trax_prefix_sum_dev(trax_ctx* ctx, const uint32_t* d_in, uint32_t
 extern "C" int trax_prefix_sum(trax_ctx* ctx, uint32_t* d_data, uint64_t n, void* stream) {
   if (n == 0) return check_ctx(ctx);
   return trax_prefix_sum_dev(ctx, d_data, d_data, n, nullptr, 0, stream);
+}
+
+extern "C" int trax_prefix_sum_segmented(trax_ctx* ctx, uint32_t* d_data, uint64_t n,
+                                         const uint32_t* d_seg_offsets, uint32_t n_segments, void* stream) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (n == 0) return TRAX_OK;
+  if (!d_data || !d_seg_offsets) return fail(ctx, TRAX_ERR_INVALID, "null scan buffer or segment offsets");
+  if (n_segments == 0) return fail(ctx, TRAX_ERR_INVALID, "a non-empty scan needs at least one segment");
+  if ((uint64_t)n_segments > trax::kStatusWords)
+    return fail(ctx, TRAX_ERR_INVALID, "more than 2^22 segments");
+  // the global scan, then each element minus the scan's value at its
+  // segment's start (gathered into the look-back words once the scan is done)
+  if ((rc = trax_prefix_sum(ctx, d_data, n, stream))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  TRAX_HIP_CHECK(ctx, trax::launch_segment_bases(d_data, n, d_seg_offsets, n_segments, ctx->status, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_segment_fix(d_data, n, d_seg_offsets, n_segments, ctx->status, s));
+  return TRAX_OK;
 }
 
 // --------------------------------------------------------------- grid build
@@ -452,7 +513,7 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals)
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -460,9 +521,9 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   const trax::SortedHits h = sorted_view(sorted);
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, ctx->ctrl, ctx->status, scan_sub(ctx),
                                              s));
+  const uint32_t items_max = pair_items_max(sp, cfg, ws);
   TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
-                                         ws->max_hits, false, ctx->flags, ctx->launch.pairs_block, s));
-  const uint32_t items_max = sp.pair_parts * ws->max_hits;
+                                         items_max, false, ctx->flags, ctx->launch.pairs_block, s));
   TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)items_max + 1,
                                             ws->work_offsets + K + 1, 1, ctx->ctrl, ctx->status, s,
                                             scan_sub(ctx)));
@@ -480,13 +541,13 @@ extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!sorted_ok(sorted) || !d_grid || !out || (!out->pairs && out->pair_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
   hipStream_t s = static_cast<hipStream_t>(stream);
   TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
                                          ws->pair_counts, reinterpret_cast<uint2*>(out->pairs),
-                                         out->pair_capacity, ws->max_hits, true, ctx->flags,
+                                         out->pair_capacity, pair_items_max(sp, cfg, ws), true, ctx->flags,
                                          ctx->launch.pairs_block, s));
   return TRAX_OK;
 }
@@ -511,7 +572,7 @@ extern "C" int trax_pairs_sector_count(trax_ctx* ctx, const trax_seed_config* cf
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals)
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -519,13 +580,13 @@ extern "C" int trax_pairs_sector_count(trax_ctx* ctx, const trax_seed_config* cf
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, ctx->ctrl, ctx->status, scan_sub(ctx),
                                              s, true));
   TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_sector_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
-                                                ws->pair_counts, nullptr, 0, ws->max_hits, false,
+                                                ws->pair_counts, nullptr, 0, sector_items_max(ws), false,
                                                 ctx->flags, s));
-  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)ws->max_hits + 1,
+  TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)sector_items_max(ws) + 1,
                                             ws->work_offsets + K, 1, ctx->ctrl, ctx->status, s,
                                             scan_sub(ctx)));
   // pair_offsets[k] = scanned count at the first inner hit of segment k
-  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->pair_counts, ws->max_hits, ws->work_offsets + K,
+  TRAX_HIP_CHECK(ctx, trax::launch_gather_offsets(ws->pair_counts, sector_items_max(ws), ws->work_offsets + K,
                                                   ws->work_offsets, K + 1, out->pair_offsets,
                                                   out->totals + 0, s));
   return TRAX_OK;
@@ -538,13 +599,13 @@ extern "C" int trax_pairs_sector_store(trax_ctx* ctx, const trax_seed_config* cf
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!sorted_ok(sorted) || !d_grid || !out || (!out->pairs && out->pair_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
   TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, static_cast<hipStream_t>(stream),
              trax::launch_sector_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets, ws->pair_counts,
                                        reinterpret_cast<uint2*>(out->pairs), out->pair_capacity,
-                                       ws->max_hits, true, ctx->flags, static_cast<hipStream_t>(stream)));
+                                       sector_items_max(ws), true, ctx->flags, static_cast<hipStream_t>(stream)));
   return TRAX_OK;
 }
 
@@ -564,7 +625,7 @@ extern "C" int trax_triplet_predict(trax_ctx* ctx, const trax_seed_config* cfg, 
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!ws->cand_stash || !ws->cand_tile_counts || (reinterpret_cast<uintptr_t>(ws->cand_tile_counts) & 15u))
     return fail(ctx, TRAX_ERR_INVALID, "workspace has no predictor stash or tile counts");
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets ||
@@ -603,7 +664,7 @@ extern "C" int trax_predict_count(trax_ctx* ctx, const trax_seed_config* cfg, ui
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets ||
       !out->totals || (!out->pairs && out->pair_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null prediction-stage buffer");
@@ -629,7 +690,7 @@ extern "C" int trax_predict_store(trax_ctx* ctx, const trax_seed_config* cfg, ui
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->totals ||
       (!out->cands && out->cand_capacity) || (!out->pairs && out->pair_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null prediction-stage buffer");
@@ -717,7 +778,7 @@ extern "C" int trax_triplet_predict_filter(trax_ctx* ctx, const trax_seed_config
   if (rc) return rc;
   trax::SeedParams sp;
   if ((rc = make_params(ctx, cfg, n_events, &sp))) return rc;
-  if ((rc = check_ws(ctx, ws, n_events, cfg->n_triplets))) return rc;
+  if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!ws->cand_stash || !ws->cand_tile_counts || !ws->trk_tile_counts ||
       (reinterpret_cast<uintptr_t>(ws->cand_tile_counts) & 15u) ||
       (reinterpret_cast<uintptr_t>(ws->trk_tile_counts) & 15u) || (reinterpret_cast<uintptr_t>(ws->cand_stash) & 7u))
@@ -761,7 +822,16 @@ extern "C" int trax_triplet_predict_filter(trax_ctx* ctx, const trax_seed_config
 extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const trax_hits* in,
                                const trax_layout* layout, trax_hits* sorted, uint32_t* d_grid,
                                const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
-  int rc = trax_grid_build(ctx, cfg, in, layout, sorted, d_grid, stream);
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!layout || !ws || !out) return fail(ctx, TRAX_ERR_INVALID, "null layout, workspace or outputs");
+  if (layout->n_hits > ws->max_hits) return fail(ctx, TRAX_ERR_INVALID, "more hits than the workspace holds");
+  {  // refuse a workspace that cannot hold the batch before anything is enqueued
+    trax::SeedParams sp;
+    if ((rc = make_params(ctx, cfg, layout->n_events, &sp))) return rc;
+    if ((rc = check_ws(ctx, ws, layout->n_events, cfg))) return rc;
+  }
+  rc = trax_grid_build(ctx, cfg, in, layout, sorted, d_grid, stream);
   if (rc) return rc;
   const uint32_t E = layout->n_events;
   rc = cfg->pair_generator == TRAX_PAIRS_SECTOR
@@ -771,6 +841,71 @@ extern "C" int trax_seed_chain(trax_ctx* ctx, const trax_seed_config* cfg, const
   if (!out->cands) return trax_triplet_predict_filter(ctx, cfg, E, sorted, d_grid, ws, out, stream);
   if ((rc = trax_triplet_predict(ctx, cfg, E, sorted, d_grid, ws, out, stream))) return rc;
   return trax_triplet_filter(ctx, cfg, E, sorted, ws, out, stream);
+}
+
+// ------------------------------------------------------------ chain graphs
+struct trax_graph {
+  trax_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int trax_chain_graph_capture(trax_ctx* ctx, const trax_seed_config* cfg, const trax_hits* in,
+                                        const trax_layout* layout, trax_hits* sorted, uint32_t* d_grid,
+                                        const trax_workspace* ws, trax_seed_outputs* out, trax_graph** graph_out) {
+  int rc = check_ctx(ctx);
+  if (rc) return rc;
+  if (!graph_out) return fail(ctx, TRAX_ERR_INVALID, "null graph handle");
+  *graph_out = nullptr;
+  hipStream_t cs = nullptr;
+  TRAX_HIP_CHECK(ctx, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  // the timer brackets eager launches only; a capture records none of its events
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return hip_fail(ctx, e, "hipStreamBeginCapture");
+  }
+  rc = trax_seed_chain(ctx, cfg, in, layout, sorted, d_grid, ws, out, cs);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (rc) {  // the chain refused its arguments: its error text stays
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(ctx, e, "hipStreamEndCapture");
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(ctx, e, "hipGraphInstantiate");
+  }
+  trax_graph* h = new trax_graph();
+  h->ctx = ctx;
+  h->graph = g;
+  h->exec = x;
+  *graph_out = h;
+  return TRAX_OK;
+}
+
+extern "C" int trax_chain_graph_launch(trax_graph* graph, void* stream) {
+  if (!graph || !graph->exec) return TRAX_ERR_INVALID;
+  int rc = check_ctx(graph->ctx);
+  if (rc) return rc;
+  TRAX_HIP_CHECK(graph->ctx, hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)));
+  return TRAX_OK;
+}
+
+extern "C" int trax_chain_graph_destroy(trax_graph* graph) {
+  if (!graph) return TRAX_OK;
+  if (graph->ctx) (void)hipSetDevice(graph->ctx->device);
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return TRAX_OK;
 }
 
 // ------------------------------------------------------------- test support

@@ -3,12 +3,16 @@
 // 240-263): GridBuilder, PairGeneratorBeamspot, TripletThetaPhiPredictor,
 // TripletThetaPhiFilter, one event group per input file.
 //
-//   trax_seed <events.bin> <results.bin> [beamspot|sector] [fused]
+//   trax_seed <events.bin> <results.bin> [beamspot|sector] [fused] [graph]
 //
 // "sector" runs PairGeneratorSector in place of PairGeneratorBeamspot;
 // "fused" runs the predictor and the filter as one pass
 // (TripletThetaPhiPredictFilter): the results then hold no candidate
-// records (count 0), only the candidate offsets.
+// records (count 0), only the candidate offsets.  "graph" then records the
+// whole chain once as a HIP graph (SeedChainGraph, capacities from the
+// stage-by-stage run) and replays it three times; every replay must equal
+// the stage-by-stage pairs, triplets and offsets, and the results written
+// are the last replay's (no candidate records, as with "fused").
 //
 // events.bin (little endian): "TRXE", u32 version=1, u32 nEvents, nLayers,
 // nHits, nSectorsZ, nSectorsPhi, nTriplets, f32 minPt, f32 minRadius[nLayers],
@@ -57,15 +61,16 @@ void put_u32(FILE* f, uint32_t v) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  bool sector = false, fused = false, bad = false;
+  bool sector = false, fused = false, graph = false, bad = false;
   for (int i = 3; i < argc; ++i) {
     const std::string a = argv[i];
     if (a == "sector") sector = true;
     else if (a == "fused") fused = true;
+    else if (a == "graph") graph = true;
     else if (a != "beamspot") bad = true;
   }
-  if (argc < 3 || argc > 5 || bad) {
-    fprintf(stderr, "usage: %s <events.bin> <results.bin> [beamspot|sector] [fused]\n", argv[0]);
+  if (argc < 3 || argc > 6 || bad) {
+    fprintf(stderr, "usage: %s <events.bin> <results.bin> [beamspot|sector] [fused] [graph]\n", argv[0]);
     return 2;
   }
   try {
@@ -146,6 +151,50 @@ int main(int argc, char** argv) {
 
     const size_t K1 = (size_t)E * P + 1;
     hipStream_t s = ctx.stream();
+    if (graph) {
+      // the whole chain as one graph, replayed three times against the
+      // stage-by-stage results
+      const std::vector<uint32_t> wantPairs = pairs->pairing.download(2 * pairs->count, s);
+      const std::vector<uint32_t> wantPairOff = pairs->pairingOffsets.download(K1, s);
+      const std::vector<uint32_t> want1 = trk->hit1.download(trk->count, s), want2 = trk->hit2.download(trk->count, s),
+                                  want3 = trk->hit3.download(trk->count, s);
+      const std::vector<uint32_t> wantOff = trk->trackletOffsets.download(K1, s);
+      const std::vector<uint32_t> wantCandOff = cands->pairingOffsets.download(K1, s);
+      auto g = std::make_unique<trax::SeedChainGraph>(ctx, hits, es, ls, grid, geom, trip, pairs->count ? pairs->count : 1,
+                                                      trk->count ? trk->count : 1,
+                                                      sector ? TRAX_PAIRS_SECTOR : TRAX_PAIRS_BEAMSPOT);
+      for (int rep = 0; rep < 3; ++rep) {
+        g->launch();
+        if (uint32_t f = g->status()) throw trax::Error("graph replay: device status flags " + std::to_string(f));
+        const uint32_t np = g->pairCount(), nt = g->tripletCount();
+        if (np != pairs->count || nt != trk->count) throw trax::Error("graph replay: different totals");
+        if (g->pairs().pairing.download(2 * (size_t)np, s) != wantPairs ||
+            g->pairs().pairingOffsets.download(K1, s) != wantPairOff ||
+            g->tracklets().hit1.download(nt, s) != want1 || g->tracklets().hit2.download(nt, s) != want2 ||
+            g->tracklets().hit3.download(nt, s) != want3 ||
+            g->tracklets().trackletOffsets.download(K1, s) != wantOff ||
+            g->candOffsets().download(K1, s) != wantCandOff)
+          throw trax::Error("graph replay " + std::to_string(rep) + " differs from the stage-by-stage run");
+      }
+      // results of the last replay (no candidate records)
+      auto gp = std::make_unique<trax::Pairing>(pairs->count, E, P);
+      auto gt = std::make_unique<trax::TrackletCollection>(trk->count, E, P);
+      auto gc = std::make_unique<trax::Pairing>(0, E, P);
+      auto copy = [&](const trax::DeviceArray<uint32_t>& from, trax::DeviceArray<uint32_t>& to, size_t n) {
+        if (n) trax::hip_check(hipMemcpyAsync(to.get(), from.get(), n * 4, hipMemcpyDeviceToDevice, s), "copy");
+      };
+      copy(g->pairs().pairing, gp->pairing, 2 * pairs->count);
+      copy(g->pairs().pairingOffsets, gp->pairingOffsets, K1);
+      copy(g->tracklets().hit1, gt->hit1, trk->count);
+      copy(g->tracklets().hit2, gt->hit2, trk->count);
+      copy(g->tracklets().hit3, gt->hit3, trk->count);
+      copy(g->tracklets().trackletOffsets, gt->trackletOffsets, K1);
+      copy(g->candOffsets(), gc->pairingOffsets, K1);
+      ctx.finish();
+      pairs = std::move(gp);
+      trk = std::move(gt);
+      cands = std::move(gc);
+    }
     FILE* out = fopen(argv[2], "wb");
     if (!out) throw trax::Error(std::string("cannot open ") + argv[2]);
     put_u32(out, 0x52585254u /* "TRXR" */);

@@ -263,8 +263,15 @@ class Workspace {
  public:
   void ensure(const trax_seed_config& cfg, uint32_t events, uint32_t hits, uint64_t pairs,
               uint64_t cands = 1) {
+    // a layout of the same sizes for this configuration: its layer-triplet
+    // count and pair work items must fit too, not only the capacities
+    trax_workspace need{};
+    if (trax_workspace_layout(&cfg, std::max(events, ws_.max_events), std::max(hits, ws_.max_hits), 1, 1, nullptr,
+                              &need) == 0)
+      throw Error("workspace layout rejected the configuration");
     if (events <= ws_.max_events && hits <= ws_.max_hits && pairs <= ws_.pair_capacity &&
-        cands <= ws_.cand_capacity && mem_.get())
+        cands <= ws_.cand_capacity && cfg.n_triplets <= ws_.n_triplets && need.pair_items <= ws_.pair_items &&
+        mem_.get())
       return;
     events = std::max(events, ws_.max_events);
     hits = std::max(hits, ws_.max_hits);
@@ -514,6 +521,74 @@ class TripletThetaPhiPredictFilter : public KernelWrapper {
  private:
   Workspace& ws;
   Totals& tot;
+};
+
+// The whole chain over one resident event group recorded as a HIP graph
+// (trax_chain_graph_capture) and replayed: for callers that seed batch
+// after batch of the same shape without the three blocking host reads of
+// the stage-by-stage classes (TripletBuilder.cpp:240-263).  Capacities are
+// fixed at construction; after a launch, status() reports an overflow and
+// the caller rebuilds with larger capacities.
+class SeedChainGraph {
+ public:
+  SeedChainGraph(Context& c, HitCollection& hits, const EventSupplement& es, const LayerSupplement& ls, Grid& grid,
+                 const GeometrySupplement& geom, const TripletConfigurations& t, uint64_t pairCapacity,
+                 uint64_t tripletCapacity, uint32_t pairGenerator = TRAX_PAIRS_BEAMSPOT)
+      : ctx(c),
+        cfg_(with_generator(make_config(grid.config, geom, t), pairGenerator)),
+        pairs_(pairCapacity, grid.config.nEvents, (uint32_t)t.size()),
+        tracklets_(tripletCapacity, grid.config.nEvents, (uint32_t)t.size()),
+        candOffsets_((size_t)grid.config.nEvents * t.size() + 1) {
+    const uint32_t E = grid.config.nEvents;
+    ws_.ensure(cfg_, E, (uint32_t)hits.size(), pairCapacity);
+    in_ = hits.input();
+    sorted_ = hits.sorted();
+    layout_ = trax_layout{E, (uint32_t)hits.size(), es.dOffset.get(), ls.dNHits.get(), ls.dOffset.get()};
+    out_ = trax_seed_outputs{};
+    out_.pairs = pairs_.pairing.get();
+    out_.pair_capacity = pairCapacity;
+    out_.pair_offsets = pairs_.pairingOffsets.get();
+    out_.cand_offsets = candOffsets_.get();  // no candidate records: the fused prediction + filter
+    out_.t1 = tracklets_.hit1.get(), out_.t2 = tracklets_.hit2.get(), out_.t3 = tracklets_.hit3.get();
+    out_.tracklet_capacity = tripletCapacity;
+    out_.tracklet_offsets = tracklets_.trackletOffsets.get();
+    out_.totals = tot_.d.get();
+    ctx.check(trax_chain_graph_capture(ctx.ctx(), &cfg_, &in_, &layout_, &sorted_, grid.boundary.get(), ws_.get(),
+                                       &out_, &graph_),
+              "trax_chain_graph_capture");
+  }
+  ~SeedChainGraph() { trax_chain_graph_destroy(graph_); }
+  SeedChainGraph(const SeedChainGraph&) = delete;
+  SeedChainGraph& operator=(const SeedChainGraph&) = delete;
+
+  void launch() { ctx.check(trax_chain_graph_launch(graph_, ctx.stream()), "trax_chain_graph_launch"); }
+  // Waits for the stream; the device status bits (0 = no overflow), cleared.
+  uint32_t status() {
+    ctx.finish();
+    return ctx.flags(true);
+  }
+  uint32_t pairCount() const { return tot_.read(ctx, 0); }
+  uint32_t tripletCount() const { return tot_.read(ctx, 2); }
+  const Pairing& pairs() const { return pairs_; }
+  const TrackletCollection& tracklets() const { return tracklets_; }
+  const DeviceArray<uint32_t>& candOffsets() const { return candOffsets_; }
+
+ private:
+  static trax_seed_config with_generator(trax_seed_config c, uint32_t g) {
+    c.pair_generator = g;
+    return c;
+  }
+  Context& ctx;
+  trax_seed_config cfg_;
+  Workspace ws_;
+  Totals tot_;
+  Pairing pairs_;
+  TrackletCollection tracklets_;
+  DeviceArray<uint32_t> candOffsets_;
+  trax_hits in_{}, sorted_{};
+  trax_layout layout_{};
+  trax_seed_outputs out_{};
+  trax_graph* graph_ = nullptr;
 };
 
 // TripletConnectivity (TripletConnectivity.h:27-56, .cpp:21-71): adds to

@@ -20,9 +20,21 @@ constexpr uint32_t kTileSlots = 512;
 uint64_t scan_tile_elems();
 // Scans of at least this many elements use the reduce-then-scan form.
 constexpr uint64_t kReduceScanMin = 1ull << 21;
+// status: look-back words (single pass: one per tile; reduce-then-scan: one
+// per chunk of 16384 elements), status_words of them.
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
                            uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
-                           hipStream_t stream, int sub = 8);
+                           hipStream_t stream, int sub = 8, uint64_t status_words = kStatusWords);
+// Elements per chunk of the reduce-then-scan form (one status word each).
+constexpr uint64_t kScanChunk = 16384;
+// Segmented fix-up after a global exclusive scan of data[0, n): bases[k] =
+// data[off[k]] (gather), then data[i] -= bases[segment of i].
+hipError_t launch_segment_bases(const uint32_t* data, uint64_t n, const uint32_t* off, uint32_t n_segments,
+                                unsigned long long* bases, hipStream_t stream);
+hipError_t launch_segment_fix(uint32_t* data, uint64_t n, const uint32_t* off, uint32_t n_segments,
+                              const unsigned long long* bases, hipStream_t stream);
+// Zeroes a scan control block (first use of a scratch block).
+hipError_t launch_scan_ctrl_clear(ScanCtrl* ctrl, hipStream_t stream);
 
 struct GridLaunch {
   const float *ix, *iy, *iz;
@@ -51,7 +63,7 @@ hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t
 // Sector pairs (count or store pass over the inner-hit work list).
 hipError_t launch_sector_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                const uint32_t* work_off, uint32_t* counts, uint2* pairs,
-                               uint64_t capacity, uint32_t max_work, bool store, uint32_t* flags,
+                               uint64_t capacity, uint32_t items_max, bool store, uint32_t* flags,
                                hipStream_t stream);
 // Beam-spot pairs: at most this many work items per outer hit
 // (== TRAX_PAIR_PARTS, checked in the C ABI); the count pass also writes the
@@ -59,7 +71,7 @@ hipError_t launch_sector_pairs(const SeedParams& sp, const SortedHits& h, const 
 constexpr uint32_t kPairParts = 2;
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                         uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
-                        uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
+                        uint32_t items_max, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,

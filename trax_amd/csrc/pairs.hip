@@ -200,11 +200,16 @@ __global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
                                                uint32_t* __restrict__ work_off,
                                                uint32_t* __restrict__ counts,  // count: out; store: scanned
                                                uint2* __restrict__ pairs, uint64_t capacity,
-                                               uint32_t* __restrict__ flags) {
+                                               uint32_t items_max, uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
-  const uint32_t parts = sp.pair_parts, W = work_off[K] * parts;  // items
+  const uint32_t parts = sp.pair_parts;
+  uint32_t W = work_off[K] * parts;  // items
+  if (W > items_max) {  // more outer hits than the counts array holds: hits beyond max_hits
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flags, TRAX_FLAG_LAYOUT);
+    W = items_max - items_max % parts;
+  }
   if (!STORE && blockIdx.x == 0 && threadIdx.x == 0) work_off[K + 1] = W;  // the scan's length - 1
   const XcdShare sh = xcd_share(W, blockDim.x, threadIdx.x);
   for (uint32_t w = sh.begin; w < sh.end; w += sh.stride) {
@@ -285,18 +290,18 @@ hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t
 
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                         uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
-                        uint32_t max_work, bool store, uint32_t* flags, uint32_t block,
+                        uint32_t items_max, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream) {
-  uint32_t blocks = (uint32_t)(((uint64_t)max_work * sp.pair_parts + block - 1) / block);
+  uint32_t blocks = (uint32_t)(((uint64_t)items_max + block - 1) / block);
   if (blocks == 0) blocks = 1;
   if (blocks > 16384u) blocks = 16384u;
   blocks = xcd_grid(blocks);
   if (store)
     hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
-                       counts, pairs, capacity, flags);
+                       counts, pairs, capacity, items_max, flags);
   else
     hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
-                       counts, pairs, capacity, flags);
+                       counts, pairs, capacity, items_max, flags);
   return hipGetLastError();
 }
 

@@ -121,7 +121,10 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
 // exclusive scan.  12 B of HBM traffic per element instead of 8, but every
 // block runs independently at streaming bandwidth, and pass 3 visits the
 // chunks in reverse order so that the ones pass 1 read last are still in
-// the Infinity Cache.
+// the Infinity Cache.  The partials live in the look-back status words, one
+// 64-bit word each with a zero upper half: a later single-pass launch then
+// reads them as "not published" (state 0) and never as a tile of its own
+// epoch, whatever the partial's value.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kChunkSub = 16;                     // uint4 per thread
 constexpr uint32_t kChunk = 256u * 4u * kChunkSub;    // elements per chunk
@@ -147,7 +150,7 @@ __device__ __forceinline__ uint4 load4(const uint32_t* in, uint64_t idx, uint64_
 
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint64_t n_max,
                                                      const uint32_t* __restrict__ d_n, uint32_t n_add,
-                                                     uint32_t* __restrict__ partials) {
+                                                     unsigned long long* __restrict__ partials) {
   __shared__ uint32_t s_wave[4];
   const uint64_t n = scan_length(n_max, d_n, n_add);
   const uint64_t base = (uint64_t)blockIdx.x * kChunk;
@@ -162,12 +165,12 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint64_
   t = wave_reduce_sum(t);
   if (lane_id() == 0) s_wave[tid >> 6] = t;
   __syncthreads();
-  if (tid == 0) partials[blockIdx.x] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+  if (tid == 0) partials[blockIdx.x] = (unsigned long long)(s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3]);
 }
 
 // Exclusive scan of the chunk partials in place (one block of 1024 threads).
 __global__ __launch_bounds__(1024) void k_scan_partials(uint64_t n_max, const uint32_t* __restrict__ d_n,
-                                                        uint32_t n_add, uint32_t* __restrict__ partials) {
+                                                        uint32_t n_add, unsigned long long* __restrict__ partials) {
   __shared__ uint32_t s_wave[16];
   const uint64_t n = scan_length(n_max, d_n, n_add);
   const uint32_t np = (uint32_t)((n + kChunk - 1) / kChunk);
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint64_t n_max, const ui
   uint32_t carry = 0;
   for (uint32_t b0 = 0; b0 < np; b0 += 1024u) {
     const uint32_t i = b0 + tid;
-    const uint32_t v = i < np ? partials[i] : 0u;
+    const uint32_t v = i < np ? (uint32_t)partials[i] : 0u;
     const uint32_t incl = wave_inclusive_scan(v);
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint64_t n_max, const ui
       wp += (w < (int)wave) ? c : 0u;
       tot += c;
     }
-    if (i < np) partials[i] = carry + wp + incl - v;
+    if (i < np) partials[i] = (unsigned long long)(carry + wp + incl - v);
     carry += tot;
     __syncthreads();
   }
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(1024) void k_scan_partials(uint64_t n_max, const ui
 
 __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n_max,
                                                    const uint32_t* __restrict__ d_n, uint32_t n_add,
-                                                   const uint32_t* __restrict__ partials) {
+                                                   const unsigned long long* __restrict__ partials) {
   __shared__ uint32_t s_wave[kChunkSub][4];
   const uint64_t n = scan_length(n_max, d_n, n_add);
   const uint32_t chunk = gridDim.x - 1u - blockIdx.x;
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
     if (lane == 63) s_wave[s][wave] = incl[s];
   }
   __syncthreads();
-  uint32_t running = partials[chunk];
+  uint32_t running = (uint32_t)partials[chunk];
 #pragma unroll
   for (uint32_t s = 0; s < kChunkSub; ++s) {
     uint32_t wp = 0, tot = 0;
@@ -241,9 +244,9 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
   }
 }
 
-// partials: ceil(n_max / kChunk) uint32 of scratch.
+// partials: ceil(n_max / kChunk) 64-bit words of scratch.
 static hipError_t scan_reduce_then_scan(const uint32_t* in, uint32_t* out, uint64_t n_max,
-                                        const uint32_t* d_n, uint32_t n_add, uint32_t* partials,
+                                        const uint32_t* d_n, uint32_t n_add, unsigned long long* partials,
                                         hipStream_t stream) {
   const uint64_t blocks = (n_max + kChunk - 1) / kChunk;
   if (blocks == 0) return hipSuccess;
@@ -255,7 +258,68 @@ static hipError_t scan_reduce_then_scan(const uint32_t* in, uint32_t* out, uint6
   return hipGetLastError();
 }
 
+static_assert(kChunk == kScanChunk, "reduce-then-scan chunk");
+
 uint64_t scan_tile_elems() { return 256ull * 4ull * 8ull; }
+
+// Segmented scan, step 2: the global exclusive scan's value at each segment
+// start (the sum of everything before the segment), one 64-bit word per
+// segment with a zero upper half (see the partials above).
+__global__ __launch_bounds__(256) void k_segment_bases(const uint32_t* __restrict__ data, uint64_t n,
+                                                       const uint32_t* __restrict__ off, uint32_t n_segments,
+                                                       unsigned long long* __restrict__ bases) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_segments; k += gridDim.x * blockDim.x) {
+    const uint32_t o = off[k];
+    bases[k] = o < n ? (unsigned long long)data[o] : 0ull;
+  }
+}
+
+// Step 3: every element minus its segment's base.  A thread takes four
+// consecutive elements: one binary search for the first, then the segment
+// index only moves forward (empty segments share their start with the next
+// one; the last segment starting at or before i holds i).
+__global__ __launch_bounds__(256) void k_segment_fix(uint32_t* __restrict__ data, uint64_t n,
+                                                     const uint32_t* __restrict__ off, uint32_t n_segments,
+                                                     const unsigned long long* __restrict__ bases) {
+  for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4u; i0 < n;
+       i0 += (uint64_t)gridDim.x * blockDim.x * 4u) {
+    uint32_t k = segment_of(off, n_segments, (uint32_t)i0);
+    for (uint64_t i = i0; i < i0 + 4u && i < n; ++i) {
+      while (k + 1u < n_segments && off[k + 1u] <= i) ++k;
+      data[i] -= (uint32_t)bases[k];
+    }
+  }
+}
+
+__global__ void k_scan_ctrl_clear(ScanCtrl* ctrl) {
+  if (threadIdx.x == 0) {
+    ctrl->ticket = 0u;
+    ctrl->done = 0u;
+    ctrl->epoch = 0u;
+    ctrl->pad = 0u;
+  }
+}
+
+hipError_t launch_segment_bases(const uint32_t* data, uint64_t n, const uint32_t* off, uint32_t n_segments,
+                                unsigned long long* bases, hipStream_t stream) {
+  uint32_t blocks = (n_segments + 255u) / 256u;
+  blocks = blocks == 0 ? 1u : (blocks > 4096u ? 4096u : blocks);
+  hipLaunchKernelGGL(k_segment_bases, dim3(blocks), dim3(256), 0, stream, data, n, off, n_segments, bases);
+  return hipGetLastError();
+}
+
+hipError_t launch_segment_fix(uint32_t* data, uint64_t n, const uint32_t* off, uint32_t n_segments,
+                              const unsigned long long* bases, hipStream_t stream) {
+  uint64_t b = (n + 1023u) / 1024u;
+  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 16384u ? 16384u : b));
+  hipLaunchKernelGGL(k_segment_fix, dim3(blocks), dim3(256), 0, stream, data, n, off, n_segments, bases);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_ctrl_clear(ScanCtrl* ctrl, hipStream_t stream) {
+  hipLaunchKernelGGL(k_scan_ctrl_clear, dim3(1), dim3(64), 0, stream, ctrl);
+  return hipGetLastError();
+}
 
 // Persistent blocks (256 CUs x 8) take tiles in ticket order; measured on
 // MI355X this beats one tile per block by 2.4x (the look-back of a block's
@@ -276,12 +340,14 @@ static hipError_t scan_launch(const uint32_t* in, uint32_t* out, uint64_t n_max,
 // sub: uint4 loads per thread per tile (tile = 1024 * sub elements).
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
                            uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
-                           hipStream_t stream, int sub) {
+                           hipStream_t stream, int sub, uint64_t status_words) {
   if (n_max >= kReduceScanMin) {
     // the status words double as the partials of the three-pass form
-    if ((n_max + kChunk - 1) / kChunk > 2 * kStatusWords) return hipErrorInvalidValue;
-    return scan_reduce_then_scan(in, out, n_max, d_n, n_add, reinterpret_cast<uint32_t*>(status), stream);
+    if ((n_max + kChunk - 1) / kChunk > status_words) return hipErrorInvalidValue;
+    return scan_reduce_then_scan(in, out, n_max, d_n, n_add, status, stream);
   }
+  // single pass: one status word per tile
+  if ((n_max + 256ull * 4ull * sub - 1) / (256ull * 4ull * sub) > status_words) return hipErrorInvalidValue;
   switch (sub) {
     case 4: return scan_launch<4>(in, out, n_max, d_n, n_add, ctrl, status, stream);
     case 16: return scan_launch<16>(in, out, n_max, d_n, n_add, ctrl, status, stream);

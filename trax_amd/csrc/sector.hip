@@ -100,11 +100,15 @@ __global__ __launch_bounds__(256) void k_sector_pairs(SeedParams sp, SortedHits 
                                                       const uint32_t* __restrict__ work_off,
                                                       uint32_t* __restrict__ counts,
                                                       uint2* __restrict__ pairs, uint64_t capacity,
-                                                      uint32_t* __restrict__ flags) {
+                                                      uint32_t items_max, uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
   const uint32_t K = sp.n_events * sp.n_triplets;
   const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
-  const uint32_t W = work_off[K];
+  uint32_t W = work_off[K];
+  if (W > items_max) {  // more inner hits than the counts array holds: hits beyond max_hits
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flags, TRAX_FLAG_LAYOUT);
+    W = items_max;
+  }
   const XcdShare sh = xcd_share(W, blockDim.x, threadIdx.x);
   for (uint32_t w = sh.begin; w < sh.end; w += sh.stride) {
     const SectorItem it = sector_item(sp, h, grid, offs, w);
@@ -130,18 +134,18 @@ __global__ __launch_bounds__(256) void k_sector_pairs(SeedParams sp, SortedHits 
 
 hipError_t launch_sector_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                const uint32_t* work_off, uint32_t* counts, uint2* pairs,
-                               uint64_t capacity, uint32_t max_work, bool store, uint32_t* flags,
+                               uint64_t capacity, uint32_t items_max, bool store, uint32_t* flags,
                                hipStream_t stream) {
-  uint32_t blocks = (max_work + 255u) / 256u;
+  uint32_t blocks = (items_max + 255u) / 256u;
   if (blocks == 0) blocks = 1;
   if (blocks > 16384u) blocks = 16384u;
   blocks = xcd_grid(blocks);
   if (store)
     hipLaunchKernelGGL(k_sector_pairs<true>, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                       grid, work_off, counts, pairs, capacity, flags);
+                       grid, work_off, counts, pairs, capacity, items_max, flags);
   else
     hipLaunchKernelGGL(k_sector_pairs<false>, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h,
-                       grid, work_off, counts, pairs, capacity, flags);
+                       grid, work_off, counts, pairs, capacity, items_max, flags);
   return hipGetLastError();
 }
 

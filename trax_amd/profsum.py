@@ -127,6 +127,7 @@ def sq(d: str, xcds: int = 8, simds: int = 1024) -> str:
         per[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
     out = ["| kernel | waves | VALU insts / wave | VMEM reads / wave | VALU issue % | active lanes % | "
            "wait % |", "|---|---|---|---|---|---|---|"]
+    machine = []
     for k, cs in per.items():
         c = {n: sum(v) / len(v) for n, v in cs.items()}
         if not c.get("SQ_WAVES") or not c.get("GRBM_GUI_ACTIVE") or not c.get("SQ_WAVE_CYCLES"):
@@ -137,7 +138,10 @@ def sq(d: str, xcds: int = 8, simds: int = 1024) -> str:
         out.append(f"| {k} | {c['SQ_WAVES']:.0f} | {c['SQ_INSTS_VALU'] / c['SQ_WAVES']:.0f} | "
                    f"{c['SQ_INSTS_VMEM_RD'] / c['SQ_WAVES']:.0f} | {100 * issue:.0f} | {100 * lanes:.0f} | "
                    f"{100 * c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:.0f} |")
-    return "\n".join(out)
+        machine.append(f"kernel={k} waves={c['SQ_WAVES']:.0f} valu_insts={c['SQ_INSTS_VALU']:.0f} "
+                       f"active_inst_valu={c['SQ_ACTIVE_INST_VALU']:.0f} gui_cycles={cycles:.0f} "
+                       f"valu_issue_frac={issue:.4f} active_lanes_frac={lanes:.4f}")
+    return "\n".join(out + [""] + machine)
 
 
 def counters(d: str, kernels=None) -> str:

@@ -70,7 +70,8 @@ class WorkspaceC(C.Structure):
                 ("max_events", C.c_uint32), ("max_hits", C.c_uint32),
                 ("pair_capacity", C.c_uint64), ("cand_capacity", C.c_uint64),
                 ("cand_stash", _vp), ("filter_mask", _vp), ("filter_counts", _vp),
-                ("cand_tile_counts", _vp), ("trk_tile_counts", _vp)]
+                ("cand_tile_counts", _vp), ("trk_tile_counts", _vp),
+                ("n_triplets", C.c_uint32), ("pair_items", C.c_uint32)]
 
 
 class TrackletsC(C.Structure):
@@ -140,6 +141,11 @@ SIGNATURES = {
                                               _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_seed_chain": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
                                   _P(WorkspaceC), _P(OutputsC), _vp]),
+    "trax_prefix_sum_segmented": (C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint32, _vp]),
+    "trax_chain_graph_capture": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
+                                           _P(WorkspaceC), _P(OutputsC), _P(_vp)]),
+    "trax_chain_graph_launch": (C.c_int, [_vp, _vp]),
+    "trax_chain_graph_destroy": (C.c_int, [_vp]),
     "trax_fp32_eval": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_uint64, _vp]),
 }
 
@@ -500,6 +506,47 @@ def prefix_sum(ctx: Context, t, stream=None) -> None:
     """In-place exclusive scan of a 16-byte aligned int32/uint32 device tensor."""
     ctx.check(ctx.lib.trax_prefix_sum(ctx.h, _vp(t.data_ptr()), t.numel(), stream_ptr(stream)),
               "trax_prefix_sum")
+
+
+def prefix_sum_segmented(ctx: Context, t, seg_offsets, stream=None) -> None:
+    """In-place exclusive scan of a device tensor that restarts at every
+    segment start; seg_offsets: device int32 tensor of n_segments + 1 offsets
+    (0 first, t.numel() last, non-decreasing)."""
+    n_seg = int(seg_offsets.numel()) - 1
+    ctx.check(ctx.lib.trax_prefix_sum_segmented(ctx.h, _vp(t.data_ptr()), t.numel(),
+                                                _vp(seg_offsets.data_ptr()), max(n_seg, 0),
+                                                stream_ptr(stream)), "trax_prefix_sum_segmented")
+
+
+class ChainGraph:
+    """The whole chain over one resident batch recorded once as a HIP graph by
+    the library (trax_chain_graph_capture) and replayed with launch()."""
+
+    def __init__(self, ctx: Context, bufs: ChainBuffers, db: DeviceBatch):
+        self.ctx = ctx
+        cfg = bufs.cfg
+        self._keep = (cfg, db.hits_c(), db.layout_c(), bufs.sorted_c(), bufs.outputs_c(), bufs, db)
+        _, hin, lay, hs, out, _, _ = self._keep
+        h = _vp()
+        ctx.check(ctx.lib.trax_chain_graph_capture(ctx.h, C.byref(cfg), C.byref(hin), C.byref(lay), C.byref(hs),
+                                                   _vp(_ptr(bufs.grid)), C.byref(bufs.ws), C.byref(out),
+                                                   C.byref(h)), "trax_chain_graph_capture")
+        self.h = h
+
+    def launch(self, stream=None) -> None:
+        ctx = self.ctx
+        ctx.check(ctx.lib.trax_chain_graph_launch(self.h, stream_ptr(stream)), "trax_chain_graph_launch")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx.lib.trax_chain_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 OPS = {"atan2": 0, "atan": 1, "acos": 2, "sign": 3, "div": 4, "sqrt": 5}

@@ -69,10 +69,14 @@ def pack_requests(reqs: Sequence["Request"], hits: np.ndarray, evt: np.ndarray, 
 
 
 class Slot:
+    max_regrow = 4  # buffer regrowths per batch before a slot gives up
+
     def __init__(self, sc: ev.SeedConfig, max_events: int, max_hits: int, pair_cap: int,
                  cand_cap: int, trk_cap: int, device="cuda"):
         import torch
         self.torch = torch
+        self.device = device
+        self.regrowths = 0
         self.stream = torch.cuda.Stream(device=device)
         self.ctx = rt.Context(torch.device(device).index or 0)
         # only the triplets are served: the chain runs the fused prediction +
@@ -152,18 +156,27 @@ class Slot:
         flags = self.ctx.flags(stream=self.stream)
         if flags & rt.FLAG_LAYOUT:
             raise rt.TraxError("inconsistent event layout")
-        if flags:
-            # outputs overflowed: grow this slot's buffers and rerun synchronously
+        for _ in range(self.max_regrow):
+            if not flags:
+                break
+            # outputs overflowed: grow this slot's buffers and rerun
+            # synchronously.  After a pair overflow the candidate and triplet
+            # totals came from truncated pairs, so the rerun may overflow a
+            # later stage in turn: hence more than one round.
             n_p, n_c, n_t = rt.totals(self.bufs)
             b = self.bufs
             self.bufs = rt.ChainBuffers(b.sc, b.max_events, b.max_hits, max(n_p, b.pair_capacity),
                                         max(n_c, b.cand_capacity), max(n_t, b.tracklet_capacity),
-                                        candidates=b.candidates)
+                                        device=self.device, candidates=b.candidates)
+            self.regrowths += 1
             with self.torch.cuda.stream(self.stream):
                 rt.seed_chain(self.ctx, self.bufs, self.db, stream=self.stream)
             self.stream.synchronize()
-            if self.ctx.flags(stream=self.stream):
-                raise rt.TraxError("slot still overflows after regrowing")
+            flags = self.ctx.flags(stream=self.stream)
+            if flags & rt.FLAG_LAYOUT:
+                raise rt.TraxError("inconsistent event layout")
+        if flags:
+            raise rt.TraxError("slot still overflows after regrowing")
         torch = self.torch
         P = len(self.bufs.sc.triplets)
         E = len(reqs)
@@ -206,14 +219,16 @@ class ServeStats:
 
 class SeedServer:
     def __init__(self, sc: ev.SeedConfig, budget_bytes: int, max_batch: int = 512, n_slots: int = 3,
-                 slot_hits: int = 1_000_000, pairs_per_hit: int = 48, device="cuda"):
+                 slot_hits: int = 1_000_000, pairs_per_hit: int = 48, device="cuda",
+                 triplets_per_pair: float = 0.25):
         self.sc = sc
         self.max_batch = max_batch
         self.budget = int(budget_bytes)
         self.slots: List[Slot] = []
         pair_cap = pairs_per_hit * slot_hits
+        trk_cap = max(1, int(pair_cap * triplets_per_pair))
         for _ in range(n_slots):
-            s = Slot(sc, max_batch, slot_hits, pair_cap, 2 * pair_cap, pair_cap // 4, device)
+            s = Slot(sc, max_batch, slot_hits, pair_cap, 2 * pair_cap, trk_cap, device)
             if sum(x.nbytes() for x in self.slots) + s.nbytes() > self.budget:
                 del s
                 break
