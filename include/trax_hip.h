@@ -218,6 +218,19 @@ int trax_prefix_sum(trax_ctx *ctx, uint32_t *d_data, uint64_t n, void *stream);
 int trax_prefix_sum_dev(trax_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint64_t n_max,
                         const uint32_t *d_n, uint32_t n_add, void *stream);
 
+/* trax_prefix_sum without a context and without allocating: no counterpart
+ * in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The scan's look-back control block and status words live in
+ * a fixed per-device array of 512 KiB + 256 B, set up on the first call on a
+ * device (the current device); n is limited to TRAX_PREFIX_SUM_NOALLOC_MAX
+ * elements (larger n: TRAX_ERR_INVALID, nothing enqueued).  d_data must be
+ * 16-byte aligned.  All calls on one device share that array: calls that
+ * overlap in time on one device (different streams or host threads) race
+ * and must be serialised by the caller. */
+#define TRAX_PREFIX_SUM_NOALLOC_MAX (1ull << 30)
+int trax_prefix_sum_noalloc(uint32_t *d_data, uint64_t n, void *stream);
+
 /* Segmented in-place exclusive scan: the scan restarts at 0 at the start of
  * every segment.  d_seg_offsets (device, n_segments + 1 entries,
  * non-decreasing, first 0, last n) gives segment k = [off[k], off[k+1]),

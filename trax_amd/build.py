@@ -21,7 +21,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TRAX_OFFLOAD_ARCH", "gfx950")
 
 DRIVERS = ["trax_seed", "trax_connect"]
-SOURCES = ["scan.hip", "grid.hip", "pairs.hip", "sector.hip", "predict.hip", "filter.hip", "connect.hip", "capi.hip"]
+SOURCES = ["scan.hip", "grid.hip", "pairs.hip", "sector.hip", "predict.hip", "filter.hip", "connect.hip", "capi.hip",
+           "noalloc.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

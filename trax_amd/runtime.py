@@ -142,6 +142,7 @@ SIGNATURES = {
     "trax_seed_chain": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
                                   _P(WorkspaceC), _P(OutputsC), _vp]),
     "trax_prefix_sum_segmented": (C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint32, _vp]),
+    "trax_prefix_sum_noalloc": (C.c_int, [_vp, C.c_uint64, _vp]),
     "trax_chain_graph_capture": (C.c_int, [_vp, _P(SeedConfigC), _P(HitsC), _P(LayoutC), _P(HitsC), _vp,
                                            _P(WorkspaceC), _P(OutputsC), _P(_vp)]),
     "trax_chain_graph_launch": (C.c_int, [_vp, _vp]),
