@@ -189,56 +189,155 @@ __device__ __forceinline__ PairItem pair_item(const SeedParams& sp, const Sorted
   return it;
 }
 
-// Count and store passes.  Work item w is part w % P of outer hit w / P,
-// P = sp.pair_parts: the z rows of the hit's window split into P consecutive
-// runs, so the items' outputs in item order are the hit's pairs in walk
-// order, and a window of many rows is walked by two lanes (the passes end
-// with their longest walks).
-template <bool STORE>
-__global__ __launch_bounds__(512) void k_pairs(SeedParams sp, SortedHits h,
-                                               const uint32_t* __restrict__ grid,
-                                               uint32_t* __restrict__ work_off,
-                                               uint32_t* __restrict__ counts,  // count: out; store: scanned
-                                               uint2* __restrict__ pairs, uint64_t capacity,
-                                               uint32_t items_max, uint32_t* __restrict__ flags) {
-  extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
-  const uint32_t K = sp.n_events * sp.n_triplets;
-  const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
+// Work item w is part w % P of outer hit w / P, P = sp.pair_parts: the z
+// rows of the hit's window split into P consecutive runs, so the items'
+// outputs in item order are the hit's pairs in walk order, and a window of
+// many rows is walked by two lanes (the passes end with their longest walks).
+__device__ __forceinline__ uint32_t pair_items(const SeedParams& sp, const uint32_t* work_off, uint32_t K,
+                                               uint32_t items_max, uint32_t* flags) {
   const uint32_t parts = sp.pair_parts;
-  uint32_t W = work_off[K] * parts;  // items
+  uint32_t W = work_off[K] * parts;
   if (W > items_max) {  // more outer hits than the counts array holds: hits beyond max_hits
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flags, TRAX_FLAG_LAYOUT);
     W = items_max - items_max % parts;
   }
-  if (!STORE && blockIdx.x == 0 && threadIdx.x == 0) work_off[K + 1] = W;  // the scan's length - 1
+  return W;
+}
+
+__device__ __forceinline__ PairItem pair_work_item(const SeedParams& sp, const SortedHits& h,
+                                                   const uint32_t* __restrict__ grid, const uint32_t* offs,
+                                                   uint32_t w) {
+  const uint32_t parts = sp.pair_parts;
+  const uint32_t hit = parts == 2u ? w >> 1 : w, part = w - hit * parts;
+  PairItem it = pair_item(sp, h, grid, offs, hit);
+  if (parts == 2u) {
+    const uint32_t lo = it.win.zLowSector, hi = it.win.zHighSector;
+    const uint32_t mid = hi > lo ? lo + (hi - lo) / 2u : lo;
+    it.win.zLowSector = part ? mid : lo;
+    it.win.zHighSector = part ? hi : mid;
+  }
+  return it;
+}
+
+// Count pass: one lane per work item walks its window.
+__global__ __launch_bounds__(512) void k_pairs_count(SeedParams sp, SortedHits h,
+                                                     const uint32_t* __restrict__ grid,
+                                                     uint32_t* __restrict__ work_off,
+                                                     uint32_t* __restrict__ counts,
+                                                     uint32_t items_max, uint32_t* __restrict__ flags) {
+  extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
+  const uint32_t K = sp.n_events * sp.n_triplets;
+  const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
+  const uint32_t W = pair_items(sp, work_off, K, items_max, flags);
+  if (blockIdx.x == 0 && threadIdx.x == 0) work_off[K + 1] = W;  // the scan's length - 1
   const XcdShare sh = xcd_share(W, blockDim.x, threadIdx.x);
   for (uint32_t w = sh.begin; w < sh.end; w += sh.stride) {
-    const uint32_t hit = parts == 2u ? w >> 1 : w, part = w - hit * parts;
-    PairItem it = pair_item(sp, h, grid, offs, hit);
-    if (parts == 2u) {
-      const uint32_t lo = it.win.zLowSector, hi = it.win.zHighSector;
-      const uint32_t mid = hi > lo ? lo + (hi - lo) / 2u : lo;
-      it.win.zLowSector = part ? mid : lo;
-      it.win.zHighSector = part ? hi : mid;
-    }
-    if (!STORE) {
-      uint32_t n = 0;
-      pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
-      counts[w] = n;
-    } else {
-      uint64_t pos = counts[w];
-      bool overflow = false;
-      pair_walk(sp, it.g1, it.win, h, [&](uint32_t idx) {
-        if (pos < capacity)
-          pairs[pos] = make_uint2(idx, it.i);
-        else
-          overflow = true;
-        ++pos;
-        return false;
-      });
-      if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
-    }
+    const PairItem it = pair_work_item(sp, h, grid, offs, w);
+    uint32_t n = 0;
+    pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
+    counts[w] = n;
   }
+}
+
+// Store pass with lane groups.  A wave's 64 work items are consecutive, and
+// so are their output runs; they are taken by sixteen groups of kStoreGroup
+// lanes, each group walking its items one after the other with its lanes
+// over consecutive inner hits of a range.  A ballot ranks the group's
+// passing hits, so the records go out in walk order as contiguous runs
+// (measured HBM write traffic 1.06x the records, against 2.4x when every
+// lane wrote its own run: partially written lines were evicted and written
+// again) and the (z, phi) loads cover consecutive hits.  Each lane first
+// evaluates its own item's window and parks it in LDS for its group.
+// PU200: 253 us against 286 us for one lane per item (eight lanes: 259 us,
+// sixteen: 302 us).
+constexpr uint32_t kStoreGroup = 4;
+static_assert(64u % kStoreGroup == 0 && kStoreGroup <= 32u, "groups tile the wave");
+
+struct StoreWindow {
+  float zLow, zHigh, phiLow, phiHigh;
+  uint32_t wrap, zLo, zHi, phiLoS, phiHiS, outer, pos;
+  uint32_t g1;  // first cell of the inner layer's grid (offset into the grid)
+};
+
+__global__ __launch_bounds__(256) void k_pairs_store(SeedParams sp, SortedHits h,
+                                                     const uint32_t* __restrict__ grid,
+                                                     const uint32_t* __restrict__ work_off,
+                                                     const uint32_t* __restrict__ counts,  // scanned
+                                                     uint2* __restrict__ pairs, uint64_t capacity,
+                                                     uint32_t items_max, uint32_t* __restrict__ flags) {
+  extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
+  __shared__ StoreWindow s_win[4][64];
+  const uint32_t K = sp.n_events * sp.n_triplets;
+  const uint32_t* offs = stage_offsets(work_off, K + 1, s_offs);
+  const uint32_t W = pair_items(sp, work_off, K, items_max, flags);
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t g = lane / kStoreGroup, sub = lane % kStoreGroup;
+  const uint32_t gmask = (1u << kStoreGroup) - 1u, below = (1u << sub) - 1u;
+  const uint32_t nPhi = sp.n_phi;
+  const XcdShare sh = xcd_share(W, blockDim.x, threadIdx.x);
+  bool overflow = false;
+  StoreWindow* wins = s_win[wave];
+  // the wave's rounds: items [wb, wb + 64), wb the same for all its lanes
+  for (uint32_t wb = sh.begin - lane; wb < sh.end; wb += sh.stride) {
+    const uint32_t w = wb + lane;
+    StoreWindow sw{};
+    if (w < sh.end) {
+      const PairItem it = pair_work_item(sp, h, grid, offs, w);
+      sw.zLow = it.win.zLow, sw.zHigh = it.win.zHigh, sw.phiLow = it.win.phiLow, sw.phiHigh = it.win.phiHigh;
+      sw.wrap = it.win.wrap ? 1u : 0u;
+      sw.zLo = it.win.zLowSector, sw.zHi = it.win.zHighSector;  // zLo == zHi: nothing to walk
+      sw.phiLoS = it.win.phiLowSector, sw.phiHiS = it.win.phiHighSector;
+      sw.outer = it.i;
+      sw.pos = counts[w];
+      sw.g1 = (uint32_t)(it.g1 - grid);
+    }
+    wins[lane] = sw;
+    wave_lds_sync();
+    for (uint32_t k = 0; k < kStoreGroup; ++k) {
+      const StoreWindow pw = wins[g * kStoreGroup + k];
+      const uint32_t* g1 = grid + pw.g1;
+      const bool wrap = pw.wrap != 0u;
+      uint64_t pos = pw.pos;
+      // inner hits [lo, hi) in index order, kStoreGroup at a time
+      auto range = [&](uint32_t lo, uint32_t hi) {
+        for (uint32_t i0 = lo; i0 < hi; i0 += kStoreGroup) {
+          const uint32_t idx = i0 + sub;
+          bool ok = false;
+          if (idx < hi) {
+            const float2 v = load_zphi(h.zphi, idx);
+            ok = pw.zLow <= v.x && v.x <= pw.zHigh && window_phi_ok(wrap, pw.phiLow, pw.phiHigh, v.y);
+          }
+          const uint32_t gb = (uint32_t)(__ballot(ok) >> (g * kStoreGroup)) & gmask;
+          if (ok) {
+            const uint64_t d = pos + (uint32_t)__popc(gb & below);
+            if (d < capacity)
+              pairs[d] = make_uint2(idx, pw.outer);
+            else
+              overflow = true;
+          }
+          pos += (uint32_t)__popc(gb);
+        }
+      };
+      // the rows and ranges of pair_walk, in the same order
+      for (uint32_t zs = pw.zLo; zs < pw.zHi; ++zs) {
+        const uint32_t row = zs * (nPhi + 1);
+        const uint32_t zSectorStart = g1[row];
+        const uint32_t zSectorEnd = g1[row + nPhi];  // padding cell excluded
+        const uint32_t len = zSectorEnd - zSectorStart;
+        const uint32_t A = g1[row + pw.phiLoS];
+        const uint32_t H = g1[row + pw.phiHiS];
+        const uint32_t E = wrap ? zSectorEnd + H - zSectorStart : H;
+        range(A, E < zSectorEnd ? E : zSectorEnd);
+        if (E > zSectorEnd) {
+          const uint32_t m = E - len;
+          range(zSectorStart, m < A ? m : A);
+          if (m > zSectorEnd) range(zSectorEnd, m);
+        }
+      }
+    }
+    wave_lds_sync();  // the windows are rewritten for the wave's next items
+  }
+  if (overflow) atomicOr(flags, TRAX_FLAG_PAIR_OVERFLOW);
 }
 
 // out[k] = scanned[min(idx[k] * mul, last)] for k in [0, n); total -> *d_total.
@@ -292,16 +391,17 @@ hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_
                         uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
                         uint32_t items_max, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream) {
-  uint32_t blocks = (uint32_t)(((uint64_t)items_max + block - 1) / block);
-  if (blocks == 0) blocks = 1;
-  if (blocks > 16384u) blocks = 16384u;
-  blocks = xcd_grid(blocks);
-  if (store)
-    hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
+  if (store) {  // 256-thread blocks (the LDS windows are laid out for four waves)
+    uint32_t blocks = (uint32_t)(((uint64_t)items_max + 255u) / 256u);
+    blocks = xcd_grid(blocks == 0 ? 1u : (blocks > 16384u ? 16384u : blocks));
+    hipLaunchKernelGGL(k_pairs_store, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
                        counts, pairs, capacity, items_max, flags);
-  else
-    hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
-                       counts, pairs, capacity, items_max, flags);
+  } else {
+    uint32_t blocks = (uint32_t)(((uint64_t)items_max + block - 1) / block);
+    blocks = xcd_grid(blocks == 0 ? 1u : (blocks > 16384u ? 16384u : blocks));
+    hipLaunchKernelGGL(k_pairs_count, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
+                       counts, items_max, flags);
+  }
   return hipGetLastError();
 }
 

@@ -37,8 +37,8 @@ from typing import Dict, List
 SHORT = [
     ("k_grid_cells", "grid_cells"),
     ("k_grid_sort", "grid_sort"),
-    ("k_pairs<false>", "pairs_count"),
-    ("k_pairs<true>", "pairs_store"),
+    ("k_pairs_count", "pairs_count"),
+    ("k_pairs_store", "pairs_store"),
     ("k_sector_pairs<false>", "sector_count"),
     ("k_sector_pairs<true>", "sector_store"),
     ("k_predict_walk", "predict_walk"),
