@@ -14,7 +14,9 @@ from typing import Dict
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# TRAX_ORACLE_LIB selects another build of the same sources (the sanitized
+# one of `make -C oracle sanitize`).
+LIB_PATH = os.environ.get("TRAX_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 
 MAX_LAYERS = 16
 MAX_TRIPLETS = 8
