@@ -44,7 +44,10 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: no plain path, no CPU baseline, a 256 MiB "
                          "copy kernel for counter calibration")
-    ap.add_argument("--tune", default=None, help="autotuning JSON to apply (trax_amd.autotune)")
+    ap.add_argument("--tune", default=None,
+                    help="autotuning JSON to apply (trax_amd.autotune); default: trax_amd/tuned/<workload>.json "
+                         "when it exists")
+    ap.add_argument("--no-tune", action="store_true", help="library default launch shapes and binning")
     ap.add_argument("--event-file", default=None,
                     help="PEventContainer file (e.g. the reference's hits.ttBar.pb) instead of synthetic events")
     ap.add_argument("--geometry", default=None,
@@ -300,11 +303,17 @@ def main():
         w.name += "-" + args.pair_generator
     log(f"{batch.n_hits} hits")
     ctx = runtime.Context(0)
-    if args.tune:
+    tune_path = args.tune
+    if tune_path is None and not args.no_tune:
+        p = os.path.join(ROOT, "trax_amd", "tuned", f"{w.name}.json")
+        tune_path = p if os.path.exists(p) else None
+    tuned = None
+    if tune_path:
         from trax_amd import autotune
-        with open(args.tune) as f:
+        with open(tune_path) as f:
             res = autotune.TuneResult(**json.load(f))
         autotune.apply(ctx, w, res)
+        tuned = {"file": os.path.relpath(tune_path, ROOT), "launch": res.launch, "binning": f"{res.n_phi}phi x {res.n_z}z"}
         log(f"tuned: {res.n_phi} phi x {res.n_z} z, launch {res.launch}")
     bufs, db = runtime.run_batch(ctx, sc, batch)
     torch.cuda.synchronize()
@@ -519,7 +528,7 @@ def main():
         "config": {"workload": w.name, "events": batch.n_events, "hits": batch.n_hits,
                    "layers": sc.n_layers, "layer_triplets": len(sc.triplets),
                    "grid": f"{sc.n_phi}phi x {sc.n_z}z", "graph": not args.no_graph,
-                   "prediction_filter": "staged" if args.unfused else "fused"},
+                   "prediction_filter": "staged" if args.unfused else "fused", "tuned": tuned},
         "triplets_per_sec": n_trk / (ms / 1e3),
         "counts": {"pairs": n_pairs, "candidates": n_cands, "triplets": n_trk},
         "plain": plain,

@@ -164,6 +164,12 @@ typedef struct {
   uint32_t pairs_block;   /* threads per block, two-phase pair kernels: 64..512 (default 256) */
   uint32_t predict_block; /* threads per block, two-phase prediction kernels (default 256) */
   uint32_t scan_tile;     /* elements per single-pass scan tile: 4096, 8192 or 16384 (default 4096) */
+  /* fused prediction + filter walk: lane buffer depth (store-window hits a
+   * lane keeps before its wave's spill pool), waves per block, and waves per
+   * SIMD the register allocation targets.  All three 0 = default (8, 4, 7);
+   * otherwise one of the compiled combinations (8, 4, 6), (8, 4, 8),
+   * (4, 4, 8), (4, 4, 7), (8, 2, 7), (12, 4, 6) */
+  uint32_t fused_buffer, fused_block_waves, fused_waves_per_eu;
 } trax_launch_config;
 
 /* ---------------------------------------------------------------- context */

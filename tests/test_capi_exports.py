@@ -73,7 +73,7 @@ def test_workspace_layout_sizes():
         assert p + sz <= q
     # the grid build needs no workspace, so the binning does not change it
     big = ev.config_workload(2, 4).seed_config
-    big.n_z, big.n_phi = 128, 256
+    big.n_z, big.n_phi = 32, 64
     cfgb = runtime.seed_config_c(big)
     nb = lib.trax_workspace_layout(ctypes.byref(cfgb), 4, 10000, 50000, 100000, ctypes.c_void_p(base),
                                    ctypes.byref(ws))

@@ -77,3 +77,34 @@ def test_autotuned_binning_matches_oracle(ctx):
     assert np.array_equal(g["tracklet_offsets"], o["tracklet_offsets"])
     assert np.array_equal(rows_sorted(with_segment(g["tracklets"], g["tracklet_offsets"])),
                           rows_sorted(with_segment(o["tracklets"], o["tracklet_offsets"])))
+
+
+def test_fused_walk_shapes_do_not_change_results(ctx):
+    """Every compiled shape of the fused prediction + filter walk (lane buffer
+    depth, waves per block, waves per SIMD) writes the same triplets,
+    offsets and totals, on pile-up events and on wide windows that fill the
+    spill pools."""
+    from trax_amd import events as ev
+    from trax_amd import runtime
+    cases = [ev.config_workload(3, 2), ev.config_workload(1, 1)]
+    for t in cases[1].seed_config.triplets:
+        t.sigma_phi, t.sigma_z = 1.5, 0.1
+    keys = ("pairs", "pair_offsets", "cand_offsets", "tracklets", "tracklet_offsets")
+    try:
+        for w in cases:
+            batch = ev.make_batch(w)
+            ref = None
+            for shape in runtime.FUSED_SHAPES:
+                print(f"{w.name}: fused shape {shape}", flush=True)
+                ctx.set_launch(fused_buffer=shape[0], fused_block_waves=shape[1], fused_waves_per_eu=shape[2])
+                bufs, _ = runtime.run_batch(ctx, w.seed_config, batch, candidates=False)
+                g = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+                if ref is None:
+                    ref = g
+                    assert len(g["tracklets"]) > 0
+                for k in keys:
+                    assert np.array_equal(g[k], ref[k]), (shape, k)
+        with pytest.raises(runtime.TraxError):
+            ctx.set_launch(fused_buffer=5, fused_block_waves=4, fused_waves_per_eu=7)
+    finally:
+        ctx.set_launch()

@@ -2,11 +2,13 @@
 
 Searches, per workload, the grid binning (phi x z sectors) and the launch
 shape of every stage (waves per grid-build block, threads per block of the
-two-phase pair and prediction kernels, scan tile size), timing each stage
-with HIP events on the launch stream.  Stages are tuned independently for a given
-binning (their launch shapes do not change their inputs); binnings are
-compared on the whole chain with each stage at its best shape.  Results are
-written as JSON and can be loaded into a context with `apply`.
+two-phase pair and prediction kernels, scan tile size, and the fused
+prediction + filter walk's lane buffer depth, waves per block and waves per
+SIMD), timing each stage with HIP events on the launch stream.  Stages are
+tuned independently for a given binning (their launch shapes do not change
+their inputs); binnings are compared on the whole chain (the fused chain the
+benchmark and the server run) with each stage at its best shape.  Results
+are written as JSON and can be loaded into a context with `apply`.
 
 The binning changes the grid and so, in edge cases, the pair and triplet
 sets (the reference's window walk depends on the cells); the parity tests
@@ -25,8 +27,12 @@ from . import runtime as rt
 WAVES = (0, 1, 2, 4, 8)  # 0: chosen from the mean segment size
 BLOCKS = (64, 128, 256, 512)
 SCAN_TILES = (4096, 8192, 16384)
+FUSED = rt.FUSED_SHAPES  # (lane buffer, waves per block, waves per SIMD)
 PHI_BINS = (64, 128, 256)
 Z_BINS = (64, 128, 256, 512)
+# the binning space per workload: the uneven pile-up-20..200 batch is tuned
+# over 64-256 phi x 32-128 z sectors
+SPACES = {2: ((64, 128, 256), (32, 64, 128))}
 
 
 @dataclass
@@ -59,9 +65,11 @@ def _time(fn, reps, torch) -> float:
 def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps: int = 3):
     """Best launch shape per stage at this binning; returns (launch, stage_ms, chain_ms)."""
     import torch
-    base = dict(grid_waves=0, pairs_block=256, predict_block=256, scan_tile=4096)
+    base = dict(grid_waves=0, pairs_block=256, predict_block=256, scan_tile=4096,
+                fused_buffer=0, fused_block_waves=0, fused_waves_per_eu=0)
     ctx.set_launch(**base)
     bufs, db = rt.run_batch(ctx, sc, batch)
+    fbufs, fdb = rt.run_batch(ctx, sc, batch, candidates=False)
     E = db.n_events
     best = dict(base)
     stage_ms: Dict[str, float] = {}
@@ -70,11 +78,17 @@ def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps:
         times = {}
         for v in values:
             cfg = dict(best)
-            cfg[key] = v
+            if isinstance(key, tuple):
+                cfg.update(zip(key, v))
+            else:
+                cfg[key] = v
             ctx.set_launch(**cfg)
             times[v] = _time(fn, reps, torch)
         v = min(times, key=times.get)
-        best[key] = v
+        if isinstance(key, tuple):
+            best.update(zip(key, v))
+        else:
+            best[key] = v
         stage_ms[name] = times[v]
         return times
 
@@ -84,11 +98,13 @@ def tune_binning(ctx: rt.Context, sc: ev.SeedConfig, batch: ev.EventBatch, reps:
           "pairs_two_phase")
     sweep("predict_block", BLOCKS,
           lambda: (rt.predict_count(ctx, bufs, E), rt.predict_store(ctx, bufs, E)), "predict_two_phase")
+    sweep(("fused_buffer", "fused_block_waves", "fused_waves_per_eu"), FUSED,
+          lambda: rt.triplet_predict_filter(ctx, fbufs, E), "predict_filter")
     stage_ms["pairs"] = _time(lambda: rt.pair_stage(ctx, bufs, E), reps, torch)
     stage_ms["predict"] = _time(lambda: rt.triplet_predict(ctx, bufs, E), reps, torch)
     ctx.set_launch(**best)
     stage_ms["filter"] = _time(lambda: rt.triplet_filter(ctx, bufs, E), reps, torch)
-    chain_ms = _time(lambda: rt.seed_chain(ctx, bufs, db), reps, torch)
+    chain_ms = _time(lambda: rt.seed_chain(ctx, fbufs, fdb), reps, torch)
     if ctx.flags():
         raise rt.TraxError("overflow during tuning")
     return best, stage_ms, chain_ms
@@ -131,14 +147,15 @@ def main():
     ap = argparse.ArgumentParser(description="autotune binning and launch shapes for a workload")
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--events", type=int, default=None)
-    ap.add_argument("--phi", type=int, nargs="*", default=list(PHI_BINS))
-    ap.add_argument("--z", type=int, nargs="*", default=list(Z_BINS))
+    ap.add_argument("--phi", type=int, nargs="*", default=None)
+    ap.add_argument("--z", type=int, nargs="*", default=None)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
     w = ev.config_workload(args.config, args.events)
+    phi_bins, z_bins = SPACES.get(args.config, (PHI_BINS, Z_BINS))
     ctx = rt.Context(0)
-    res = tune(ctx, w, phi_bins=args.phi, z_bins=args.z, reps=args.reps,
+    res = tune(ctx, w, phi_bins=args.phi or phi_bins, z_bins=args.z or z_bins, reps=args.reps,
                log=lambda m: print(m, flush=True))
     with open(args.out, "w") as f:
         f.write(res.to_json())

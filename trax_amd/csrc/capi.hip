@@ -25,7 +25,8 @@ struct trax_ctx {
   uint32_t* flags = nullptr;
   unsigned long long* status = nullptr;
   size_t lds_max = 65536;
-  trax_launch_config launch = {0, 256, 256, 4096};
+  trax_launch_config launch = {0, 256, 256, 4096, 0, 0, 0};
+  uint32_t fused_variant = 0;  // index into trax::kFusedVariants
   // kernel timer: HIP events recorded on the launch stream around every
   // eager launch of one selected kernel
   int timed_kernel = 0;
@@ -345,7 +346,20 @@ extern "C" int trax_ctx_set_launch(trax_ctx* ctx, const trax_launch_config* cfg)
       return fail(ctx, TRAX_ERR_INVALID, "block sizes must be 64, 128, 256 or 512");
   if (!(cfg->scan_tile == 4096 || cfg->scan_tile == 8192 || cfg->scan_tile == 16384))
     return fail(ctx, TRAX_ERR_INVALID, "scan_tile must be 4096, 8192 or 16384");
+  uint32_t variant = 0;
+  if (cfg->fused_buffer || cfg->fused_block_waves || cfg->fused_waves_per_eu) {
+    variant = trax::kNumFusedVariants;
+    for (uint32_t v = 0; v < trax::kNumFusedVariants; ++v) {
+      const trax::FusedVariant& f = trax::kFusedVariants[v];
+      if (f.buffer == cfg->fused_buffer && f.block_waves == cfg->fused_block_waves &&
+          f.waves_per_eu == cfg->fused_waves_per_eu)
+        variant = v;
+    }
+    if (variant == trax::kNumFusedVariants)
+      return fail(ctx, TRAX_ERR_INVALID, "fused walk shape is not one of the compiled combinations");
+  }
   ctx->launch = *cfg;
+  ctx->fused_variant = variant;
   return TRAX_OK;
 }
 
@@ -798,7 +812,7 @@ extern "C" int trax_triplet_predict_filter(trax_ctx* ctx, const trax_seed_config
   uint2* slab = reinterpret_cast<uint2*>(ws->cand_stash);
   TRAX_TIMED(ctx, TRAX_KERNEL_PREDICT_FILTER, s,
              trax::launch_predict_filter_walk(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
-                                              ws->cand_counts, tiles, slab, acc, s));
+                                              ws->cand_counts, tiles, slab, acc, s, ctx->fused_variant));
   TRAX_HIP_CHECK(ctx, trax::launch_acc_redo(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
                                             ws->cand_counts, slab, acc, s));
   TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s,

@@ -100,10 +100,18 @@ hipError_t launch_predict_place(const SeedParams& sp, const SortedHits& h, const
 // Prediction and filter in one pass: the walk (candidate budgets, tile
 // totals, accepted records per tile in the slab), then after the scans the
 // triplet offsets and the triplets.
+// Compiled launch shapes of the fused walk: lane buffer depth, waves per
+// block, waves per SIMD targeted by the register allocation.
+struct FusedVariant {
+  uint32_t buffer, block_waves, waves_per_eu;
+};
+constexpr uint32_t kNumFusedVariants = 7;
+extern const FusedVariant kFusedVariants[kNumFusedVariants];
 hipError_t launch_predict_filter_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                       const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                                       uint64_t pair_capacity, uint32_t* counts, uint32_t* tile_counts,
-                                      uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream);
+                                      uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream,
+                                      uint32_t variant = 0);
 hipError_t launch_acc_redo(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
                            const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
                            const uint32_t* counts, uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream);

@@ -540,17 +540,20 @@ __device__ __forceinline__ uint32_t queue_cut(uint16_t* queue, uint32_t nq, Cut&
   return n_out;
 }
 
-// Per-wave LDS of the fused walk.  A lane buffers its first kFuseBuf
+// Per-wave LDS of the fused walk.  A lane buffers its first BUF
 // store-window hits in its own column; later ones go to the wave's spill
 // pool (pool order is each lane's slot order).  Queue entries are
 // (j << 6 | lane) for buffered slot j, kSpillTag + k for pool entry k.
-constexpr uint32_t kFuseBuf = 8;
+// BUF (the lane buffer depth), the waves per block and the waves per SIMD
+// the register allocation targets are tuning knobs (trax_launch_config);
+// the compiled combinations are listed in kFusedVariants.
 constexpr uint32_t kSpill = 192;
 constexpr uint32_t kSpillTag = 1024;
-static_assert(kFuseBuf * 64 <= kSpillTag && kSpillTag + kSpill <= 65536, "queue entries are 16-bit");
+template <uint32_t BUF>
 struct FuseLds {
-  uint32_t buf[kFuseBuf * 64];  // buf[j * 64 + lane]
-  uint16_t queue[kFuseBuf * 64 + kSpill];
+  static_assert(BUF * 64 <= kSpillTag && kSpillTag + kSpill <= 65536 && BUF <= 32, "queue entries are 16-bit");
+  uint32_t buf[BUF * 64];  // buf[j * 64 + lane]
+  uint16_t queue[BUF * 64 + kSpill];
   uint32_t acc[64];             // accept bits of buffered slots, per lane
   uint32_t spill_hit[kSpill];
   uint16_t spill_tag[kSpill];   // j << 6 | lane
@@ -563,9 +566,10 @@ struct FuseLds {
 // filled slots, and the tile's accepted total and slab records (or per-pair
 // counts when more than kAccSlots are accepted) are written.  Returns false,
 // leaving the tile to k_acc_redo, when the spill pool overflows.
+template <uint32_t BUF>
 __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                            const uint2* pairs, const uint32_t* offs, uint32_t nP, uint32_t t,
-                                           FuseLds& L, uint32_t* counts, uint32_t* tile_counts, uint2* acc_slab,
+                                           FuseLds<BUF>& L, uint32_t* counts, uint32_t* tile_counts, uint2* acc_slab,
                                            uint32_t* acc_tiles) {
   const uint32_t lane = lane_id();
   const uint32_t q = t * kPredTile + lane;
@@ -576,7 +580,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   PredItem it{};
   if (q < nP) {
     it = pred_item(sp, h, grid, pairs, offs, q);
-    n_count = predict_count_buffer<kFuseBuf, 64>(sp, h, it, s_col, &n_store, [&](uint32_t j, uint32_t idx) {
+    n_count = predict_count_buffer<BUF, 64>(sp, h, it, s_col, &n_store, [&](uint32_t j, uint32_t idx) {
       const uint32_t k = atomicAdd(&L.spill_n, 1u);
       if (k < kSpill) {
         L.spill_hit[k] = idx;
@@ -593,7 +597,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   const uint32_t ns = L.spill_n;
   if (ns > kSpill) return false;
   // the queue: buffered slots, then pool entries within their pair's budget
-  const uint32_t nb = n_eff < kFuseBuf ? n_eff : kFuseBuf;
+  const uint32_t nb = n_eff < BUF ? n_eff : BUF;
   const uint32_t qincl = wave_inclusive_scan(nb);
   uint32_t nq = __shfl(qincl, 63, 64);
   for (uint32_t j = 0; j < nb; ++j) L.queue[qincl - nb + j] = (uint16_t)((j << 6) | lane);
@@ -676,10 +680,12 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   return true;
 }
 
-// Seven waves per SIMD: at most 72 VGPRs (two values spill to scratch) and
-// under 22.8 KB of LDS per block; PU200: 1.22 ms against 1.31 ms at the six
-// the unconstrained register allocation (79 VGPRs) allows, 1.26 ms at eight.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_predict_filter_walk(SeedParams sp, SortedHits h,
+// Default: an 8-deep lane buffer, four waves per block, seven waves per
+// SIMD (at most 72 VGPRs, under 22.8 KB of LDS per block); PU200: 1.22 ms
+// against 1.31 ms at the six the unconstrained register allocation (79
+// VGPRs) allows, 1.26 ms at eight.
+template <uint32_t BUF, uint32_t NW, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void k_predict_filter_walk(SeedParams sp, SortedHits h,
                                                              const uint32_t* __restrict__ grid,
                                                              const uint2* __restrict__ pairs,
                                                              const uint32_t* __restrict__ pair_off,
@@ -690,15 +696,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                                                              uint2* __restrict__ acc_slab,
                                                              uint32_t* __restrict__ acc_tiles) {
   extern __shared__ uint32_t s_offs[];
-  __shared__ FuseLds s_lds[4];
+  __shared__ FuseLds<BUF> s_lds[NW];
   const uint32_t K = sp.n_events * sp.n_triplets, tid = threadIdx.x, wave = tid >> 6;
   const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
   const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
-  const XcdShare sh = xcd_share(ntiles, 4u, wave);
+  const XcdShare sh = xcd_share(ntiles, NW, wave);
   for (uint32_t t = sh.begin; t < sh.end; t += sh.stride) {
-    if (!fused_tile(sp, h, grid, pairs, offs, nP, t, s_lds[wave], counts, tile_counts, acc_slab, acc_tiles) &&
+    if (!fused_tile<BUF>(sp, h, grid, pairs, offs, nP, t, s_lds[wave], counts, tile_counts, acc_slab, acc_tiles) &&
         lane_id() == 0)
       acc_tiles[t] = kTileRedo;  // left to k_acc_redo
   }
@@ -877,13 +883,41 @@ __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, 
   if (overflow) atomicOr(flags, TRAX_FLAG_TRACKLET_OVERFLOW);
 }
 
+// The compiled (lane buffer, waves per block, waves per SIMD) combinations
+// of the fused walk; entry 0 is the default.
+const FusedVariant kFusedVariants[kNumFusedVariants] = {
+    {8, 4, 7}, {8, 4, 6}, {8, 4, 8}, {4, 4, 8}, {4, 4, 7}, {8, 2, 7}, {12, 4, 6}};
+
+template <uint32_t BUF, uint32_t NW, int WPE>
+static void fused_launch(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
+                         const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
+                         uint32_t* counts, uint32_t* tile_counts, uint2* acc_slab, uint32_t* acc_tiles,
+                         hipStream_t stream) {
+  const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
+  uint64_t b = (tiles + NW - 1) / NW;  // NW waves (tiles) per block and round
+  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 65536u / NW ? 65536u / NW : b));
+  hipLaunchKernelGGL((k_predict_filter_walk<BUF, NW, WPE>), dim3(xcd_grid(blocks)), dim3(64 * NW),
+                     offsets_lds_bytes(sp), stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts,
+                     tile_counts, acc_slab, acc_tiles);
+}
+
 hipError_t launch_predict_filter_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                       const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                                       uint64_t pair_capacity, uint32_t* counts, uint32_t* tile_counts,
-                                      uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream) {
-  hipLaunchKernelGGL(k_predict_filter_walk, dim3(xcd_grid(tile_blocks(pair_capacity))), dim3(256), offsets_lds_bytes(sp),
-                     stream, sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, tile_counts, acc_slab,
-                     acc_tiles);
+                                      uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream, uint32_t variant) {
+#define TRAX_FUSED_CASE(i, B, W, E) \
+  case i: fused_launch<B, W, E>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, tile_counts, acc_slab, acc_tiles, stream); break;
+  switch (variant) {
+    TRAX_FUSED_CASE(1, 8, 4, 6)
+    TRAX_FUSED_CASE(2, 8, 4, 8)
+    TRAX_FUSED_CASE(3, 4, 4, 8)
+    TRAX_FUSED_CASE(4, 4, 4, 7)
+    TRAX_FUSED_CASE(5, 8, 2, 7)
+    TRAX_FUSED_CASE(6, 12, 4, 6)
+    default: fused_launch<8, 4, 7>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, tile_counts,
+                                   acc_slab, acc_tiles, stream);
+  }
+#undef TRAX_FUSED_CASE
   return hipGetLastError();
 }
 

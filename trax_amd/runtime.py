@@ -81,7 +81,14 @@ class TrackletsC(C.Structure):
 
 class LaunchC(C.Structure):
     _fields_ = [("grid_waves", C.c_uint32), ("pairs_block", C.c_uint32),
-                ("predict_block", C.c_uint32), ("scan_tile", C.c_uint32)]
+                ("predict_block", C.c_uint32), ("scan_tile", C.c_uint32),
+                ("fused_buffer", C.c_uint32), ("fused_block_waves", C.c_uint32),
+                ("fused_waves_per_eu", C.c_uint32)]
+
+
+# Compiled (lane buffer, waves per block, waves per SIMD) shapes of the
+# fused prediction + filter walk (trax_launch_config); the first is the default.
+FUSED_SHAPES = ((8, 4, 7), (8, 4, 6), (8, 4, 8), (4, 4, 8), (4, 4, 7), (8, 2, 7), (12, 4, 6))
 
 
 class OutputsC(C.Structure):
@@ -233,8 +240,10 @@ class Context:
             msg = self.lib.trax_ctx_last_error(self.h)
             raise TraxError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def set_launch(self, grid_waves=0, pairs_block=256, predict_block=256, scan_tile=4096):
-        cfg = LaunchC(grid_waves, pairs_block, predict_block, scan_tile)
+    def set_launch(self, grid_waves=0, pairs_block=256, predict_block=256, scan_tile=4096,
+                   fused_buffer=0, fused_block_waves=0, fused_waves_per_eu=0):
+        cfg = LaunchC(grid_waves, pairs_block, predict_block, scan_tile, fused_buffer, fused_block_waves,
+                      fused_waves_per_eu)
         self.check(self.lib.trax_ctx_set_launch(self.h, C.byref(cfg)), "trax_ctx_set_launch")
 
     def get_launch(self) -> dict:
