@@ -795,7 +795,7 @@ extern "C" int trax_triplet_predict_filter(trax_ctx* ctx, const trax_seed_config
   if ((rc = check_ws(ctx, ws, n_events, cfg))) return rc;
   if (!ws->cand_stash || !ws->cand_tile_counts || !ws->trk_tile_counts ||
       (reinterpret_cast<uintptr_t>(ws->cand_tile_counts) & 15u) ||
-      (reinterpret_cast<uintptr_t>(ws->trk_tile_counts) & 15u) || (reinterpret_cast<uintptr_t>(ws->cand_stash) & 7u))
+      (reinterpret_cast<uintptr_t>(ws->trk_tile_counts) & 15u) || (reinterpret_cast<uintptr_t>(ws->cand_stash) & 15u))
     return fail(ctx, TRAX_ERR_INVALID, "workspace has no predictor stash or tile counts");
   if (!sorted_ok(sorted) || !d_grid || !out || !out->pair_offsets || !out->cand_offsets || !out->totals ||
       !out->tracklet_offsets || (!out->pairs && out->pair_capacity) ||
@@ -809,7 +809,7 @@ extern "C" int trax_triplet_predict_filter(trax_ctx* ctx, const trax_seed_config
   const uint64_t tiles_max = (pc + trax::kPredTile - 1) / trax::kPredTile;
   uint32_t* tiles = ws->cand_tile_counts;
   uint32_t* acc = ws->trk_tile_counts;
-  uint2* slab = reinterpret_cast<uint2*>(ws->cand_stash);
+  uint4* slab = reinterpret_cast<uint4*>(ws->cand_stash);
   TRAX_TIMED(ctx, TRAX_KERNEL_PREDICT_FILTER, s,
              trax::launch_predict_filter_walk(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
                                               ws->cand_counts, tiles, slab, acc, s, ctx->fused_variant));

@@ -110,17 +110,17 @@ extern const FusedVariant kFusedVariants[kNumFusedVariants];
 hipError_t launch_predict_filter_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                       const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                                       uint64_t pair_capacity, uint32_t* counts, uint32_t* tile_counts,
-                                      uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream,
+                                      uint4* acc_slab, uint32_t* acc_tiles, hipStream_t stream,
                                       uint32_t variant = 0);
 hipError_t launch_acc_redo(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
                            const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
-                           const uint32_t* counts, uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream);
-hipError_t launch_acc_offsets(const uint32_t* acc_scanned, const uint2* acc_slab, const uint32_t* pair_off,
+                           const uint32_t* counts, uint4* acc_slab, uint32_t* acc_tiles, hipStream_t stream);
+hipError_t launch_acc_offsets(const uint32_t* acc_scanned, const uint4* acc_slab, const uint32_t* pair_off,
                               const uint32_t* d_npairs, uint64_t pair_capacity, uint32_t n, uint32_t* trk_off,
                               uint32_t* d_total, hipStream_t stream);
 hipError_t launch_acc_place(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
                             const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
-                            const uint32_t* acc_scanned, const uint32_t* counts, const uint2* acc_slab,
+                            const uint32_t* acc_scanned, const uint32_t* counts, const uint4* acc_slab,
                             uint32_t* t1, uint32_t* t2, uint32_t* t3, uint64_t trk_capacity, uint32_t* flags,
                             hipStream_t stream);
 // Filter: mask pass, scan of the word counts, store pass.

@@ -474,7 +474,9 @@ hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const 
 // are accepted, the per-pair accepted counts.  The candidate budgets still
 // give the candidate offsets and total.  After the prefix sum of the
 // accepted tile totals, pass 2 writes the triplets.
-constexpr uint32_t kAccSlots = kTileSlots / 2;  // uint2 records per tile slab
+// A tile's slab holds its accepted records as (pair, hit1, hit2, hit3), so
+// that placing the triplets needs no pair-record gather.
+constexpr uint32_t kAccSlots = kTileSlots / 4;  // uint4 records per tile slab
 constexpr uint32_t kTileRedo = 0xFFFFFFFFu;     // accepted total not known yet
 
 // Accepted store-window hits of pair `it` among its slots [skip, budget),
@@ -569,7 +571,7 @@ struct FuseLds {
 template <uint32_t BUF>
 __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                            const uint2* pairs, const uint32_t* offs, uint32_t nP, uint32_t t,
-                                           FuseLds<BUF>& L, uint32_t* counts, uint32_t* tile_counts, uint2* acc_slab,
+                                           FuseLds<BUF>& L, uint32_t* counts, uint32_t* tile_counts, uint4* acc_slab,
                                            uint32_t* acc_tiles) {
   const uint32_t lane = lane_id();
   const uint32_t q = t * kPredTile + lane;
@@ -667,12 +669,16 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   const uint32_t aincl = wave_inclusive_scan(a);
   const uint32_t A = __shfl(aincl, 63, 64);
   if (lane == 0) acc_tiles[t] = A;
-  uint2* slab = acc_slab + (uint64_t)t * kAccSlots;
+  uint4* slab = acc_slab + (uint64_t)t * kAccSlots;
   if (A <= kAccSlots) {
     uint32_t pos = aincl - a;
-    for (uint32_t m = mask; m; m &= m - 1u) slab[pos++] = make_uint2(q, s_col[(uint32_t)__builtin_ctz(m) * 64u]);
-    for (uint32_t k = 0; k < ns; ++k)
-      if (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) slab[pos++] = make_uint2(q, L.spill_hit[k]);
+    if (a) {
+      const uint2 pr = pairs[q];  // read again: keeping it through the walk costs registers
+      for (uint32_t m = mask; m; m &= m - 1u)
+        slab[pos++] = make_uint4(q, pr.x, pr.y, s_col[(uint32_t)__builtin_ctz(m) * 64u]);
+      for (uint32_t k = 0; k < ns; ++k)
+        if (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) slab[pos++] = make_uint4(q, pr.x, pr.y, L.spill_hit[k]);
+    }
   } else {
     reinterpret_cast<uint32_t*>(slab)[lane] = a;  // pass 2 walks this tile's pairs again
   }
@@ -693,7 +699,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
                                                              uint64_t pair_capacity,
                                                              uint32_t* __restrict__ counts,
                                                              uint32_t* __restrict__ tile_counts,
-                                                             uint2* __restrict__ acc_slab,
+                                                             uint4* __restrict__ acc_slab,
                                                              uint32_t* __restrict__ acc_tiles) {
   extern __shared__ uint32_t s_offs[];
   __shared__ FuseLds<BUF> s_lds[NW];
@@ -729,7 +735,7 @@ __global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, co
                                                  const uint32_t* __restrict__ pair_off,
                                                  const uint32_t* __restrict__ d_npairs, uint64_t pair_capacity,
                                                  const uint32_t* __restrict__ counts,
-                                                 uint2* __restrict__ acc_slab, uint32_t* __restrict__ acc_tiles) {
+                                                 uint4* __restrict__ acc_slab, uint32_t* __restrict__ acc_tiles) {
   const uint32_t lane = lane_id();
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
@@ -750,10 +756,13 @@ __global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, co
       }
       const uint32_t aincl = wave_inclusive_scan(a);
       const uint32_t A = __shfl(aincl, 63, 64);
-      uint2* slab = acc_slab + (uint64_t)t * kAccSlots;
+      uint4* slab = acc_slab + (uint64_t)t * kAccSlots;
       if (A <= kAccSlots) {
         uint32_t pos = aincl - a;
-        if (a) accept_walk(sp, h, it, 0u, n, [&](uint32_t idx) { slab[pos++] = make_uint2(q, idx); });
+        if (a) {
+          const uint2 pr = pairs[q];
+          accept_walk(sp, h, it, 0u, n, [&](uint32_t idx) { slab[pos++] = make_uint4(q, pr.x, pr.y, idx); });
+        }
       } else {
         reinterpret_cast<uint32_t*>(slab)[lane] = a;
       }
@@ -767,7 +776,7 @@ __global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, co
 // pair's tile, the records of lower pairs (slab records are in pair order)
 // or the per-pair counts of an overflowed slab.  The last entry is the total.
 __global__ __launch_bounds__(256) void k_acc_offsets(const uint32_t* __restrict__ acc_scanned,
-                                                     const uint2* __restrict__ acc_slab,
+                                                     const uint4* __restrict__ acc_slab,
                                                      const uint32_t* __restrict__ pair_off,
                                                      const uint32_t* __restrict__ d_npairs,
                                                      uint64_t pair_capacity, uint32_t n,
@@ -782,7 +791,7 @@ __global__ __launch_bounds__(256) void k_acc_offsets(const uint32_t* __restrict_
     uint32_t s = acc_scanned[t];
     if (lq) {  // t is a tile with pairs
       const uint32_t A = acc_scanned[t + 1u] - s;
-      const uint2* slab = acc_slab + (uint64_t)t * kAccSlots;
+      const uint4* slab = acc_slab + (uint64_t)t * kAccSlots;
       if (A <= kAccSlots) {
         for (uint32_t j = 0; j < A && slab[j].x < q; ++j) ++s;
       } else {
@@ -809,7 +818,7 @@ __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, 
                                                    const uint32_t* __restrict__ d_npairs, uint64_t pair_capacity,
                                                    const uint32_t* __restrict__ acc_scanned,
                                                    const uint32_t* __restrict__ counts,
-                                                   const uint2* __restrict__ acc_slab, uint32_t* __restrict__ t1,
+                                                   const uint4* __restrict__ acc_slab, uint32_t* __restrict__ t1,
                                                    uint32_t* __restrict__ t2, uint32_t* __restrict__ t3,
                                                    uint64_t trk_capacity, uint32_t* __restrict__ flags) {
   __shared__ uint32_t s_incl[4][kPlaceTiles];
@@ -844,13 +853,12 @@ __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, 
         if (incl_w[mid] > r) hi = mid; else lo = mid + 1u;
       }
       const uint32_t j = r - (lo ? incl_w[lo - 1u] : 0u);
-      const uint2 rec = acc_slab[(uint64_t)(t0 + lo) * kAccSlots + j];
-      const uint2 pr = pairs[rec.x];
+      const uint4 rec = acc_slab[(uint64_t)(t0 + lo) * kAccSlots + j];  // (pair, hit1, hit2, hit3)
       const uint64_t d = (uint64_t)base_w[lo] + j;
       if (d < trk_capacity) {
-        t1[d] = pr.x;
-        t2[d] = pr.y;
-        t3[d] = rec.y;
+        t1[d] = rec.y;
+        t2[d] = rec.z;
+        t3[d] = rec.w;
       } else {
         overflow = true;
       }
@@ -891,7 +899,7 @@ const FusedVariant kFusedVariants[kNumFusedVariants] = {
 template <uint32_t BUF, uint32_t NW, int WPE>
 static void fused_launch(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
                          const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
-                         uint32_t* counts, uint32_t* tile_counts, uint2* acc_slab, uint32_t* acc_tiles,
+                         uint32_t* counts, uint32_t* tile_counts, uint4* acc_slab, uint32_t* acc_tiles,
                          hipStream_t stream) {
   const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
   uint64_t b = (tiles + NW - 1) / NW;  // NW waves (tiles) per block and round
@@ -904,7 +912,7 @@ static void fused_launch(const SeedParams& sp, const SortedHits& h, const uint32
 hipError_t launch_predict_filter_walk(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                       const uint2* pairs, const uint32_t* pair_off, const uint32_t* d_npairs,
                                       uint64_t pair_capacity, uint32_t* counts, uint32_t* tile_counts,
-                                      uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream, uint32_t variant) {
+                                      uint4* acc_slab, uint32_t* acc_tiles, hipStream_t stream, uint32_t variant) {
 #define TRAX_FUSED_CASE(i, B, W, E) \
   case i: fused_launch<B, W, E>(sp, h, grid, pairs, pair_off, d_npairs, pair_capacity, counts, tile_counts, acc_slab, acc_tiles, stream); break;
   switch (variant) {
@@ -923,7 +931,7 @@ hipError_t launch_predict_filter_walk(const SeedParams& sp, const SortedHits& h,
 
 hipError_t launch_acc_redo(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
                            const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
-                           const uint32_t* counts, uint2* acc_slab, uint32_t* acc_tiles, hipStream_t stream) {
+                           const uint32_t* counts, uint4* acc_slab, uint32_t* acc_tiles, hipStream_t stream) {
   // one wave per block, one lane per tile: 64 tiles per block and round
   const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
   uint64_t b = (tiles + 63u) / 64u;
@@ -933,7 +941,7 @@ hipError_t launch_acc_redo(const SeedParams& sp, const SortedHits& h, const uint
   return hipGetLastError();
 }
 
-hipError_t launch_acc_offsets(const uint32_t* acc_scanned, const uint2* acc_slab, const uint32_t* pair_off,
+hipError_t launch_acc_offsets(const uint32_t* acc_scanned, const uint4* acc_slab, const uint32_t* pair_off,
                               const uint32_t* d_npairs, uint64_t pair_capacity, uint32_t n, uint32_t* trk_off,
                               uint32_t* d_total, hipStream_t stream) {
   uint32_t blocks = (n + 255u) / 256u;
@@ -945,7 +953,7 @@ hipError_t launch_acc_offsets(const uint32_t* acc_scanned, const uint2* acc_slab
 
 hipError_t launch_acc_place(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
                             const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
-                            const uint32_t* acc_scanned, const uint32_t* counts, const uint2* acc_slab,
+                            const uint32_t* acc_scanned, const uint32_t* counts, const uint4* acc_slab,
                             uint32_t* t1, uint32_t* t2, uint32_t* t3, uint64_t trk_capacity, uint32_t* flags,
                             hipStream_t stream) {
   // kPlaceTiles tiles per block and round
