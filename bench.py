@@ -174,14 +174,36 @@ def cpu_baseline(sc, batch, threads: int, reps: int = 5, single_sample: int = 16
     return batch.n_events / dt, res, dt, one, m, runs
 
 
-def limiter(kernel: str, issue):
+def measured_ta(workload: str, kernel: str):
+    """Average-CU texture-address busy share of `kernel` from the committed
+    TA counter summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"ta_busy_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 3 and cells[0] == kernel:
+                try:
+                    return float(cells[2]) / 100.0
+                except ValueError:
+                    return None
+    return None
+
+
+def limiter(kernel: str, issue, ta, traffic_frac):
     """What bounds the dominant kernel, in words, from its counters."""
-    if kernel in ("predict_filter", "predict_walk", "filter_mask", "pairs") :
-        txt = ("instruction issue and gather latency, not HBM: per-lane window walks "
-               "(dependent (z, phi) gathers, then compares) and the cuts' FP32 arithmetic")
+    if kernel in ("predict_filter", "predict_walk", "filter_mask", "pairs"):
+        txt = ("the mix of dependent gathers and VALU work, not HBM: per-lane window walks "
+               "((z, phi) gathers after grid-cell loads, then compares) and the cuts' FP32 arithmetic")
+        parts = []
         if issue:
-            txt += f"; VALU issue slots used: {100 * issue['valu_issue_frac']:.0f}%"
-        return txt
+            parts.append(f"VALU issue {100 * issue['valu_issue_frac']:.0f}% of the measured peak")
+        if ta is not None:
+            parts.append(f"texture-address units busy {100 * ta:.0f}%")
+        if traffic_frac is not None:
+            parts.append(f"HBM {100 * traffic_frac:.1f}% of 8 TB/s")
+        return txt + ("; " + ", ".join(parts) if parts else "")
     return "HBM"
 
 
@@ -512,6 +534,7 @@ def main():
     dom = kern[dominant]
     traffic = measured_traffic(w.name, dominant)
     issue = measured_issue(w.name, dominant)
+    ta_busy = measured_ta(w.name, dominant)
     out = {
         "metric": "events/sec (batched seeding chain, 1 GPU)",
         "value": ev_s,
@@ -553,8 +576,11 @@ def main():
                      "traffic_gbs": traffic / (dom["ms"] * 1e-3) / 1e9 if traffic else None,
                      "traffic_frac": traffic / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "ms": dom["ms"], "algorithmic_bytes": dom["bytes"],
-                     "limiter": limiter(dominant, issue),
-                     "valu_issue": issue},
+                     "limiter": limiter(dominant, issue, ta_busy, traffic / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                        if traffic else None),
+                     "valu_issue": dict(issue, peak_wave_insts_per_simd_cycle=0.474,
+                                        source=f"profiles/valu_issue_{w.name}.md") if issue else None,
+                     "ta_busy": ta_busy},
         "cpu_baseline": cpu,
         "connectivity": conn,
         "copy_calibration": copy,

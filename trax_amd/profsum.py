@@ -7,9 +7,12 @@
            SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_WAVE_CYCLES SQ_BUSY_CYCLES
            SQ_WAIT_INST_ANY SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU
            GRBM_GUI_ACTIVE: VALU instructions per wave, the share of the
-           SIMDs' VALU issue slots used (one wave64 VALU instruction per SIMD
-           every 4 cycles), the share of lanes active per VALU instruction,
-           and the share of wave time spent waiting on dependencies.
+           SIMDs' measured peak VALU issue rate used (VALU_PEAK_PER_SIMD_CYCLE),
+           the share of lanes active per VALU instruction, and the share of
+           wave time spent waiting on dependencies.
+`ta`     : texture-address unit busy share and SALU / LDS instructions per
+           wave from one counter run with TA_BUSY_avr TA_BUSY_max
+           GRBM_GUI_ACTIVE SQ_INSTS_SALU SQ_INSTS_LDS SQ_WAVES.
 `counters`: per-kernel averages of every counter of one counter run, e.g.
            `--pmc OccupancyPercent LdsBankConflict LdsUtil SQ_WAVES` (wave
            occupancy, LDS bank-conflict cycles per LDS access, LDS busy %).
@@ -120,12 +123,20 @@ def _counter_means(d: str, counter: str) -> Dict[str, float]:
     return {k: sum(v) / len(v) for k, v in per.items()}
 
 
-def sq(d: str, xcds: int = 8, simds: int = 1024) -> str:
+# Wave64 FP32 VALU instructions one SIMD issues per GRBM_GUI_ACTIVE cycle at
+# most, measured on MI355X with a kernel of independent multiplies (32 per
+# loop iteration, four independent chains, 8192 blocks of 256 threads):
+# 2.1e9 wave instructions in 2.03 ms over 1024 SIMDs and 4.32 M GUI cycles
+# per XCD, i.e. one instruction per SIMD about every 2.1 cycles.
+VALU_PEAK_PER_SIMD_CYCLE = 0.474
+
+
+def sq(d: str, xcds: int = 8, simds: int = 1024, valu_rate: float = VALU_PEAK_PER_SIMD_CYCLE) -> str:
     rows = _rows(_one(os.path.join(d, "**", "*counter_collection.csv")))
     per: Dict[str, Dict[str, List[float]]] = defaultdict(lambda: defaultdict(list))
     for r in rows:
         per[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
-    out = ["| kernel | waves | VALU insts / wave | VMEM reads / wave | VALU issue % | active lanes % | "
+    out = ["| kernel | waves | VALU insts / wave | VMEM reads / wave | VALU issue % of peak | active lanes % | "
            "wait % |", "|---|---|---|---|---|---|---|"]
     machine = []
     for k, cs in per.items():
@@ -133,7 +144,10 @@ def sq(d: str, xcds: int = 8, simds: int = 1024) -> str:
         if not c.get("SQ_WAVES") or not c.get("GRBM_GUI_ACTIVE") or not c.get("SQ_WAVE_CYCLES"):
             continue
         cycles = c["GRBM_GUI_ACTIVE"] / xcds  # GRBM_GUI_ACTIVE sums the XCDs
-        issue = c["SQ_ACTIVE_INST_VALU"] / (simds * cycles / 4.0)
+        # SQ_ACTIVE_INST_VALU counts issued VALU instructions (it equals
+        # SQ_INSTS_VALU on the probe kernel), so the share of the peak issue
+        # rate is instructions / (SIMDs x cycles x peak rate)
+        issue = c["SQ_ACTIVE_INST_VALU"] / (simds * cycles * valu_rate)
         lanes = c["SQ_THREAD_CYCLES_VALU"] / (64.0 * c["SQ_ACTIVE_INST_VALU"]) if c["SQ_ACTIVE_INST_VALU"] else 0
         out.append(f"| {k} | {c['SQ_WAVES']:.0f} | {c['SQ_INSTS_VALU'] / c['SQ_WAVES']:.0f} | "
                    f"{c['SQ_INSTS_VMEM_RD'] / c['SQ_WAVES']:.0f} | {100 * issue:.0f} | {100 * lanes:.0f} | "
@@ -142,6 +156,28 @@ def sq(d: str, xcds: int = 8, simds: int = 1024) -> str:
                        f"active_inst_valu={c['SQ_ACTIVE_INST_VALU']:.0f} gui_cycles={cycles:.0f} "
                        f"valu_issue_frac={issue:.4f} active_lanes_frac={lanes:.4f}")
     return "\n".join(out + [""] + machine)
+
+
+def ta(d: str, xcds: int = 8, tas: int = 256) -> str:
+    """Texture-address unit busy share and SALU / LDS instructions per wave
+    from one counter run with TA_BUSY_avr TA_BUSY_max GRBM_GUI_ACTIVE
+    SQ_INSTS_SALU SQ_INSTS_LDS SQ_WAVES (TA_BUSY_avr: busy cycles averaged
+    over the TA instances, one per CU)."""
+    rows = _rows(_one(os.path.join(d, "**", "*counter_collection.csv")))
+    per: Dict[str, Dict[str, List[float]]] = defaultdict(lambda: defaultdict(list))
+    for r in rows:
+        per[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = ["| kernel | waves | TA busy % (avg CU) | TA busy % (busiest CU) | SALU insts / wave | LDS insts / wave |",
+           "|---|---|---|---|---|---|"]
+    for k, cs in per.items():
+        c = {n: sum(v) / len(v) for n, v in cs.items()}
+        if not c.get("SQ_WAVES") or not c.get("GRBM_GUI_ACTIVE"):
+            continue
+        cycles = c["GRBM_GUI_ACTIVE"] / xcds
+        out.append(f"| {k} | {c['SQ_WAVES']:.0f} | {100 * c.get('TA_BUSY_avr', 0) / cycles:.0f} | "
+                   f"{100 * c.get('TA_BUSY_max', 0) / cycles:.0f} | {c.get('SQ_INSTS_SALU', 0) / c['SQ_WAVES']:.0f} | "
+                   f"{c.get('SQ_INSTS_LDS', 0) / c['SQ_WAVES']:.0f} |")
+    return "\n".join(out)
 
 
 def counters(d: str, kernels=None) -> str:
@@ -202,6 +238,9 @@ def main():
     c = sub.add_parser("sq")
     c.add_argument("dir")
     c.add_argument("--command", default="")
+    t = sub.add_parser("ta")
+    t.add_argument("dir")
+    t.add_argument("--command", default="")
     d = sub.add_parser("counters")
     d.add_argument("dir")
     d.add_argument("--kernels", default="", help="comma-separated short names (default: all)")
@@ -218,6 +257,8 @@ def main():
         print(stats(args.dir))
     elif args.what == "sq":
         print(sq(args.dir))
+    elif args.what == "ta":
+        print(ta(args.dir))
     elif args.what == "counters":
         print(counters(args.dir, [k for k in args.kernels.split(",") if k] or None))
     else:
