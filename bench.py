@@ -98,11 +98,11 @@ ALGO_BYTES = {
     # read and three hit indexes written
     "filter_store": lambda c: 12 * c["cands"] // 64 + 28 * c["triplets"],
     # fused prediction + filter walk: the walk's bytes without the stash
-    # slots, x, y, z of each candidate's third hit read, one uint2 record
-    # per accepted triplet written
+    # slots, x, y, z of each candidate's third hit read, one 16-byte
+    # (pair, hit1, hit2, hit3) record per accepted triplet written
     "predict_filter": lambda c: 16 * c["hits"] + 4 * c["cells"] + 12 * c["pairs"] + 12 * c["cands"]
-                                + 8 * c["triplets"],
-    # per triplet its record and pair read and three hit indexes written
+                                + 16 * c["triplets"],
+    # per triplet its 16-byte record read and three hit indexes written
     "triplet_place": lambda c: 28 * c["triplets"],
 }
 

@@ -7,7 +7,8 @@
  * the given HIP stream (`void*` = hipStream_t; NULL = default stream) and
  * returns an int status (TRAX_OK or a negative TRAX_ERR_*); the text of the
  * last error is kept per context.  Nothing here allocates device memory
- * except trax_ctx_create; callers own every buffer.  A context serialises its
+ * except trax_ctx_create and trax_chain_graph_capture (the graph object);
+ * callers own every buffer.  A context serialises its
  * launches on one stream at a time (as the reference does with its one
  * in-order OpenCL queue): use one context per concurrent stream.
  *
