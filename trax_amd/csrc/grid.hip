@@ -6,19 +6,17 @@
 // E*L*C grid, and gridStore (scatter with atomic_inc ranks, so the order
 // inside a cell is non-deterministic).
 //
-// Here two kernels, no host read and no global scan:
-//
-// k_grid_cells, elementwise over the batch: phi = atan2(y, x) and the cell of
-// every hit, parked in input order in the phi and orig_index output columns.
-// The transcendental and division work is spread over the whole device
-// instead of over one block per segment.
-//
-// k_grid_sort, one block per (event, layer) segment, everything in LDS:
+// Here one kernel, no host read and no global scan.  k_grid_sort takes one
+// (event, layer) segment per block, everything in LDS:
+// * each thread loads its hits' columns (segments of up to 64 * waves * HPT
+//   hits keep all of them in registers from the first load to the last
+//   store, so the block waits for memory once), computes phi = atan2(y, x)
+//   and the cell, and the cells go to LDS in input order;
 // * the hits' cells are counted over the segment's occupied cell range
 //   [cmin, cmax] (a few thousand cells for a barrel layer); the exclusive
 //   scan of the counts is the segment's grid over that range, and below /
-//   above it the grid is the segment base / base + hit count, so the block
-//   writes the whole grid of its segment with coalesced stores;
+//   above it the grid is the segment base / base + hit count, filled with
+//   16-byte stores that drain while the sort works;
 // * each hit takes a slot of its cell (LDS atomics) and its final position is
 //   the cell start plus the number of same-cell hits with a smaller input
 //   index, so equal cells keep input order;
@@ -27,9 +25,9 @@
 //   index) keys (5-bit digits; ranks from 64-lane ballot matches and
 //   per-(digit, wave) counters over consecutive input ranges) and the grid
 //   from binary searches in the sorted keys;
-// * the columns are permuted through LDS: each column is read coalesced in
-//   input order (the next pair is in flight while the current one is
-//   stored) and written coalesced in sorted order.
+// * the columns are permuted through LDS two per round: written in input
+//   order (from registers, or read coalesced for larger segments), gathered
+//   by sorted position and stored coalesced.
 // Segments of more than 8192 hits sort (cell, index) pairs with the same
 // radix passes in place in four of the output columns before those are
 // overwritten.
@@ -51,8 +49,7 @@ struct GridArgs {
   const uint32_t* __restrict__ idet;
   const uint32_t* __restrict__ ihid;
   const uint32_t* __restrict__ ievt;
-  // outputs; ophi/oorig hold (phi, cell) in input order between the cell and
-  // sort kernels, odet/ohid/oevt/olayer are sort scratch for large segments
+  // outputs; odet/ohid/oevt/olayer are sort scratch for large segments
   float* ox;
   float* oy;
   float* oz;
@@ -75,7 +72,6 @@ constexpr uint32_t kKeyIdxBits = 13;   // index bits of a packed LDS key
 constexpr uint32_t kLdsSortMax = 1u << kKeyIdxBits;
 constexpr uint32_t kDigitBits = 5;
 constexpr uint32_t kDigits = 1u << kDigitBits;
-constexpr uint32_t kCellThreads = 256;
 constexpr uint32_t kOccupancyMax = 32;  // counting sort: most hits in one cell
 
 __device__ __forceinline__ void segment_range(const GridArgs& a, uint32_t seg, uint32_t L,
@@ -187,16 +183,6 @@ __device__ __forceinline__ void radix_pass(uint32_t n, Load load, Digit digit, S
   __syncthreads();
 }
 
-// Elementwise: (phi, cell) of every hit of the batch, in input order.
-__global__ __launch_bounds__(kCellThreads) void k_grid_cells(SeedParams sp, GridArgs a) {
-  const uint32_t n = a.n_hits;
-  for (uint32_t i = blockIdx.x * kCellThreads + threadIdx.x; i < n; i += gridDim.x * kCellThreads) {
-    const float phi = trax_atan2f(a.iy[i], a.ix[i]);
-    a.ophi[i] = phi;
-    a.oorig[i] = grid_cell_zphi(sp, a.iz[i], phi);
-  }
-}
-
 // Block-wide min of lo and max of hi (every thread gets both).
 template <int NW>
 __device__ __forceinline__ void block_min_max(uint32_t lo, uint32_t hi, uint32_t* s_wave,
@@ -240,13 +226,30 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* S, uint32_t 
   return lo;
 }
 
+// p[0..n) = v by the block: 16-byte stores between a head and a tail of at
+// most three words each.
+template <uint32_t BLOCK>
+__device__ __forceinline__ void fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+  static_assert(BLOCK >= 4, "head and tail take one store per thread");
+  const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(p) >> 2) & 3u);
+  uint32_t head = mis ? 4u - mis : 0u;
+  head = head < n ? head : n;
+  if (threadIdx.x < head) p[threadIdx.x] = v;
+  uint4* q = reinterpret_cast<uint4*>(p + head);
+  const uint32_t nq = (n - head) >> 2;
+  const uint4 vv = make_uint4(v, v, v, v);
+  for (uint32_t c = threadIdx.x; c < nq; c += BLOCK) q[c] = vv;
+  const uint32_t done = head + nq * 4u;
+  if (threadIdx.x < n - done) p[done + threadIdx.x] = v;
+}
+
 // Grid entries of a segment outside its occupied cell range [cmin, cmax]:
-// the segment base below it, base + nh above it (coalesced stores).
+// the segment base below it, base + nh above it.
 template <uint32_t BLOCK>
 __device__ __forceinline__ void grid_constant_parts(uint32_t* g, uint32_t C, uint32_t cmin,
                                                     uint32_t cmax, uint32_t base, uint32_t nh) {
-  for (uint32_t c = threadIdx.x; c < cmin && c < C; c += BLOCK) g[c] = base;
-  for (uint32_t c = cmax + 1u + threadIdx.x; c < C; c += BLOCK) g[c] = base + nh;
+  fill_u32<BLOCK>(g, cmin < C ? cmin : C, base);
+  if (cmax + 1u < C) fill_u32<BLOCK>(g + cmax + 1u, C - (cmax + 1u), base + nh);
 }
 
 // In-block exclusive scan of cnt[0..n) that also returns the largest count.
@@ -296,11 +299,21 @@ __device__ __forceinline__ uint32_t block_scan_occupancy(uint32_t* cnt, uint32_t
   return occ;
 }
 
-// Per-hit state lives in LDS, so the kernel needs few registers and two
-// blocks fit on a CU: every segment of a 64-event batch is resident from the
-// start.
-template <int NW>
-__global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a, uint32_t lds_cap) {
+// The hit columns of a segment, HPT hits per thread (hit tid + k * BLOCK),
+// held in registers from the block's first load to its last store.
+template <int HPT>
+struct HitRegs {
+  float x[HPT], y[HPT], z[HPT], phi[HPT];
+  uint32_t layer[HPT], det[HPT], hid[HPT], evt[HPT];
+};
+
+// One block per (event, layer) segment at a time.  Segments of at most
+// 64 * NW * HPT hits keep every column in registers: their loads are issued
+// once, at the start, and the permutation rounds only move data through LDS.
+// Larger segments read the columns round by round (up to the LDS cap) or
+// sort in place in four output columns.
+template <int NW, int HPT, int WPE>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void k_grid_sort(SeedParams sp, GridArgs a, uint32_t lds_cap) {
   constexpr uint32_t BLOCK = 64u * NW;
   extern __shared__ uint32_t s_lds[];  // R0, R1: lds_cap words each; R2: lds_cap uint16
   __shared__ uint32_t s_cnt[kDigits * NW];
@@ -316,20 +329,51 @@ __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a
     uint32_t base, nh;
     segment_extent(a, seg, L, &base, &nh, true);
     uint32_t* g = a.grid + (uint64_t)seg * C;
+    if (nh == 0) {
+      fill_u32<BLOCK>(g, C, base);
+      continue;
+    }
     if (nh <= cap) {
-      // cells in input order into R1; the segment's occupied cell range
+      // every hit's phi = atan2(y, x) and cell; cells in input order into R1;
+      // the segment's occupied cell range
+      const bool regs = nh <= BLOCK * (uint32_t)HPT;  // block-uniform
+      HitRegs<HPT> v;
       uint32_t cmin = 0xFFFFFFFFu, cmax = 0u;
-      for (uint32_t i = tid; i < nh; i += BLOCK) {
-        const uint32_t c = a.oorig[base + i];
-        cmin = c < cmin ? c : cmin;
-        cmax = c > cmax ? c : cmax;
-        R1[i] = c;
+      if (regs) {
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+          const uint32_t i = tid + (uint32_t)k * BLOCK;
+          if (i < nh) {
+            v.x[k] = a.ix[base + i];
+            v.y[k] = a.iy[base + i];
+            v.z[k] = a.iz[base + i];
+            v.layer[k] = a.ilayer[base + i];
+            v.det[k] = a.idet[base + i];
+            v.hid[k] = a.ihid[base + i];
+            v.evt[k] = a.ievt[base + i];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+          const uint32_t i = tid + (uint32_t)k * BLOCK;
+          if (i < nh) {
+            v.phi[k] = trax_atan2f(v.y[k], v.x[k]);
+            const uint32_t c = grid_cell_zphi(sp, v.z[k], v.phi[k]);
+            cmin = c < cmin ? c : cmin;
+            cmax = c > cmax ? c : cmax;
+            R1[i] = c;
+          }
+        }
+      } else {
+        for (uint32_t i = tid; i < nh; i += BLOCK) {
+          const float phi = trax_atan2f(a.iy[base + i], a.ix[base + i]);
+          const uint32_t c = grid_cell_zphi(sp, a.iz[base + i], phi);
+          cmin = c < cmin ? c : cmin;
+          cmax = c > cmax ? c : cmax;
+          R1[i] = c;
+        }
       }
       block_min_max<NW>(cmin, cmax, s_wave, &cmin, &cmax);
-      if (nh == 0) {
-        for (uint32_t c = tid; c < C; c += BLOCK) g[c] = base;
-        continue;
-      }
       const uint32_t span = cmax - cmin + 1u;  // cells <= 2^24
       uint32_t span_bits = 0;
       while (((span - 1u) >> span_bits) != 0u) ++span_bits;
@@ -359,8 +403,6 @@ __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a
               R1[i] = (x << kKeyIdxBits) | slot;
             }
             __syncthreads();
-            // final position: cell start + same-cell hits with a smaller
-            // index (R0[x] is now the end of cell x, R0[x - 1] its start)
             for (uint32_t i = tid; i < nh; i += BLOCK) {
               const uint32_t x = R1[i] >> kKeyIdxBits;
               const uint32_t st = x ? R0[x - 1u] : 0u, en = R0[x];
@@ -370,8 +412,8 @@ __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a
             }
             __syncthreads();
             for (uint32_t i = tid; i < nh; i += BLOCK) {
-              const uint32_t v = R1[i];
-              R0[v & (kLdsSortMax - 1u)] = (v & ~(kLdsSortMax - 1u)) | i;
+              const uint32_t w = R1[i];
+              R0[w & (kLdsSortMax - 1u)] = (w & ~(kLdsSortMax - 1u)) | i;
             }
             __syncthreads();
             S = R0;
@@ -402,22 +444,42 @@ __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a
             g[cmin + x] = base + lower_bound_u32(S, nh, x << kKeyIdxBits);
         }
         // Columns in sorted order, two per round: the sorted input indexes
-        // move to the 16-bit region R2, the columns are read coalesced in
-        // input order into R0/R1, gathered by sorted position and stored
-        // coalesced.  Each column is read completely before it is written,
-        // so the input-order phi parked in the phi column is safe.
+        // move to the 16-bit region R2, the columns go in input order into
+        // R0/R1 (from registers, or read coalesced), are gathered by sorted
+        // position and stored coalesced.
         for (uint32_t p = tid; p < nh; p += BLOCK) R2[p] = (uint16_t)(S[p] & (kLdsSortMax - 1u));
         __syncthreads();
-        const uint32_t* col_a[4] = {reinterpret_cast<const uint32_t*>(a.ix),
-                                    reinterpret_cast<const uint32_t*>(a.iz), a.ilayer, a.ihid};
-        const uint32_t* col_b[4] = {reinterpret_cast<const uint32_t*>(a.iy),
-                                    reinterpret_cast<const uint32_t*>(a.ophi), a.idet, a.ievt};
         for (uint32_t r = 0; r < 4; ++r) {
-          const uint32_t* ca = col_a[r] + base;
-          const uint32_t* cb = col_b[r] + base;
-          for (uint32_t i = tid; i < nh; i += BLOCK) {
-            R0[i] = ca[i];
-            R1[i] = cb[i];
+          if (regs) {
+#pragma unroll
+            for (int k = 0; k < HPT; ++k) {
+              const uint32_t i = tid + (uint32_t)k * BLOCK;
+              if (i < nh) {
+                R0[i] = r == 0 ? __builtin_bit_cast(uint32_t, v.x[k])
+                        : r == 1 ? __builtin_bit_cast(uint32_t, v.z[k])
+                        : r == 2 ? v.layer[k] : v.hid[k];
+                R1[i] = r == 0 ? __builtin_bit_cast(uint32_t, v.y[k])
+                        : r == 1 ? __builtin_bit_cast(uint32_t, v.phi[k])
+                        : r == 2 ? v.det[k] : v.evt[k];
+              }
+            }
+          } else {
+            for (uint32_t i = tid; i < nh; i += BLOCK) {
+              const uint32_t s = base + i;
+              if (r == 0) {
+                R0[i] = __builtin_bit_cast(uint32_t, a.ix[s]);
+                R1[i] = __builtin_bit_cast(uint32_t, a.iy[s]);
+              } else if (r == 1) {
+                R0[i] = __builtin_bit_cast(uint32_t, a.iz[s]);
+                R1[i] = __builtin_bit_cast(uint32_t, trax_atan2f(a.iy[s], a.ix[s]));
+              } else if (r == 2) {
+                R0[i] = a.ilayer[s];
+                R1[i] = a.idet[s];
+              } else {
+                R0[i] = a.ihid[s];
+                R1[i] = a.ievt[s];
+              }
+            }
           }
           __syncthreads();
           for (uint32_t p = tid; p < nh; p += BLOCK) {
@@ -453,7 +515,8 @@ __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a
     uint32_t* cell_b = a.oevt + base;
     uint32_t* idx_b = a.olayer + base;
     for (uint32_t i = tid; i < nh; i += BLOCK) {
-      cell_a[i] = a.oorig[base + i];
+      const uint32_t s = base + i;
+      cell_a[i] = grid_cell_zphi(sp, a.iz[s], trax_atan2f(a.iy[s], a.ix[s]));
       idx_a[i] = i;
     }
     __syncthreads();
@@ -483,8 +546,7 @@ __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a
     __syncthreads();
     for (uint32_t p0 = 0; p0 < nh; p0 += BLOCK) {
       // read this round's indexes before any thread overwrites the scratch
-      // columns; phi is recomputed because its input-order copy is
-      // overwritten round by round
+      // columns
       const uint32_t p = p0 + tid;
       const bool valid = p < nh;
       const uint32_t idx = valid ? idx_a[p] : 0u;
@@ -509,10 +571,22 @@ __global__ __launch_bounds__(64 * NW) void k_grid_sort(SeedParams sp, GridArgs a
   }
 }
 
+// Hits per thread kept in registers, per block size: the PU200 and uneven
+// batches' largest segments (about 4,100 hits) fit 512 threads x 9.  Waves
+// per SIMD the register allocation must allow: two 512-thread blocks per CU.
+template <int NW>
+constexpr int grid_hpt() {
+  return NW == 8 ? 9 : 4;
+}
+template <int NW>
+constexpr int grid_wpe() {
+  return NW >= 8 ? 4 : 1;
+}
+
 template <int NW>
 static void grid_sort_launch(uint32_t blocks, size_t lds, uint32_t cap, hipStream_t stream,
                              const SeedParams& sp, const GridArgs& a) {
-  hipLaunchKernelGGL((k_grid_sort<NW>), dim3(blocks), dim3(64 * NW), lds, stream, sp, a, cap);
+  hipLaunchKernelGGL((k_grid_sort<NW, grid_hpt<NW>(), grid_wpe<NW>()>), dim3(blocks), dim3(64 * NW), lds, stream, sp, a, cap);
 }
 
 // Static LDS of a sort block (counters, wave partials) stays below this; a
@@ -533,13 +607,6 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
   a.flags = gl.flags;
   const uint32_t nseg = sp.n_events * sp.n_layers;
   if (nseg == 0) return hipSuccess;
-  if (gl.n_hits > 0) {
-    const uint32_t want = (gl.n_hits + kCellThreads - 1u) / kCellThreads;
-    const uint32_t blocks = want < 8192u ? want : 8192u;
-    hipLaunchKernelGGL(k_grid_cells, dim3(blocks), dim3(kCellThreads), 0, stream, sp, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
   // 10 bytes of LDS per hit (two word regions, one 16-bit region), sized so
   // that two blocks share a CU
   const size_t budget = gl.lds_max < kSortLdsPerBlock ? gl.lds_max : kSortLdsPerBlock;
@@ -556,13 +623,14 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
   if (fit < cap) cap = (uint32_t)fit;
   const size_t lds = (size_t)(cap > 0 ? cap : 64u) * 10u;
   const uint32_t blocks = nseg < 65535u ? nseg : 65535u;
-  // automatic block size: about four hits of the mean segment per thread
-  // (1 wave for the few-hundred-hit segments of single collisions, 8 for
-  // high pile-up)
+  // automatic block size: at most two hits of the mean segment per thread,
+  // so that segments of several times the mean keep their columns in
+  // registers (1 wave for the few-hundred-hit segments of single
+  // collisions, 8 from 1,024 hits on)
   int waves = gl.waves;
   if (waves == 0) {
     const uint32_t per = (gl.n_hits / nseg) / 256u;
-    waves = per >= 8u ? 8 : per >= 4u ? 4 : per >= 2u ? 2 : 1;
+    waves = per >= 4u ? 8 : per >= 2u ? 4 : per >= 1u ? 2 : 1;
   }
   switch (waves) {
     case 1: grid_sort_launch<1>(blocks, lds, cap, stream, sp, a); break;
@@ -576,11 +644,11 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
 
 hipError_t grid_build_set_lds_limit(size_t bytes) {
   hipError_t e = hipSuccess;
-  const void* fns[] = {reinterpret_cast<const void*>(&k_grid_sort<1>),
-                       reinterpret_cast<const void*>(&k_grid_sort<2>),
-                       reinterpret_cast<const void*>(&k_grid_sort<4>),
-                       reinterpret_cast<const void*>(&k_grid_sort<8>),
-                       reinterpret_cast<const void*>(&k_grid_sort<16>)};
+  const void* fns[] = {reinterpret_cast<const void*>(&k_grid_sort<1, grid_hpt<1>(), grid_wpe<1>()>),
+                       reinterpret_cast<const void*>(&k_grid_sort<2, grid_hpt<2>(), grid_wpe<2>()>),
+                       reinterpret_cast<const void*>(&k_grid_sort<4, grid_hpt<4>(), grid_wpe<4>()>),
+                       reinterpret_cast<const void*>(&k_grid_sort<8, grid_hpt<8>(), grid_wpe<8>()>),
+                       reinterpret_cast<const void*>(&k_grid_sort<16, grid_hpt<16>(), grid_wpe<16>()>)};
   const size_t dyn = bytes > kSortStaticLds ? bytes - kSortStaticLds : 0u;
   const size_t want = kSortLdsPerBlock - kSortStaticLds;
   for (const void* f : fns) {

@@ -38,7 +38,7 @@ from typing import Dict, List
 
 # Chain kernels by the short names the library's kernel timer uses.
 SHORT = [
-    ("k_grid_cells", "grid_cells"),
+    ("k_grid_cells", "grid_cells"),  # earlier builds
     ("k_grid_sort", "grid_sort"),
     ("k_pairs_count", "pairs_count"),
     ("k_pairs_store", "pairs_store"),
@@ -222,10 +222,12 @@ def traffic(fetch_dir: str, write_dir: str, workload: str) -> str:
         total[k] = fb + wb
         lines.append(f"kernel={k} fetch_kib={fetch[k]:.1f} write_kib={write[k]:.1f} "
                      f"hbm_bytes={fb + wb:.0f}")
-    # the grid stage is two kernels (cells, then the per-segment sort)
-    if "grid_cells" in total and "grid_sort" in total:
-        lines.append(f"kernel=grid_build hbm_bytes={total['grid_cells'] + total['grid_sort']:.0f} "
-                     f"(grid_cells + grid_sort)")
+    # the grid stage is the per-segment sort (earlier builds ran a separate
+    # cell kernel before it)
+    if "grid_sort" in total:
+        parts = [k for k in ("grid_cells", "grid_sort") if k in total]
+        lines.append(f"kernel=grid_build hbm_bytes={sum(total[k] for k in parts):.0f} "
+                     f"({' + '.join(parts)})")
     return "\n".join(lines)
 
 
