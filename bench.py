@@ -458,7 +458,7 @@ def main():
     # the input columns and layout up, the triplet columns and offsets down)
     host_in = {k: t.cpu().pin_memory() for k, t in db.cols.items()}
     h2d_ms = time_events(lambda: [db.cols[k].copy_(host_in[k], non_blocking=True) for k in host_in],
-                         5, torch)
+                         20, torch)
     host_out = [torch.empty(n_trk, dtype=torch.int32, pin_memory=True) for _ in range(3)]
     host_off = torch.empty(bufs.tracklet_offsets.numel(), dtype=torch.int32, pin_memory=True)
 
@@ -466,7 +466,7 @@ def main():
         for i in range(3):
             host_out[i].copy_(bufs.t[i][:n_trk], non_blocking=True)
         host_off.copy_(bufs.tracklet_offsets, non_blocking=True)
-    d2h_ms = time_events(d2h, 5, torch)
+    d2h_ms = time_events(d2h, 20, torch)
     in_bytes = sum(t.numel() * t.element_size() for t in host_in.values())
     pcie = {"h2d_ms": h2d_ms, "d2h_ms": d2h_ms, "h2d_bytes": in_bytes,
             "d2h_bytes": 12 * n_trk + host_off.numel() * 4,

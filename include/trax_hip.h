@@ -375,7 +375,7 @@ int trax_seed_chain(trax_ctx *ctx, const trax_seed_config *cfg, const trax_hits 
  * shape again and again: trax_chain_graph_capture records trax_seed_chain
  * with these exact arguments (buffers, capacities, configuration) into a
  * graph and instantiates it; trax_chain_graph_launch replays it on `stream`
- * (one graph launch instead of ~14 kernel launches, no host work per
+ * (one graph launch instead of ~13 kernel launches, no host work per
  * stage).  It replaces the reference's stage sequence with its three
  * blocking host reads (PairGeneratorBeamspot.cpp:120,
  * TripletThetaPhiPredictor.cpp:59, TripletThetaPhiFilter.cpp:82; called
