@@ -200,6 +200,7 @@ int trax_ctx_flags(trax_ctx *ctx, void *stream, uint32_t *flags_out, int clear);
 #define TRAX_KERNEL_CONNECT 8         /* triplet connectivity: table clear, insert, lookup */
 #define TRAX_KERNEL_PREDICT_FILTER 9  /* fused prediction + filter walk */
 #define TRAX_KERNEL_TRIPLET_PLACE 10  /* fused path: accepted records -> triplets */
+#define TRAX_KERNEL_ACC_CUT 11        /* fused path: theta and circle cuts over the phi survivors */
 int trax_ctx_kernel_timer(trax_ctx *ctx, int kernel);
 int trax_ctx_kernel_time(trax_ctx *ctx, double *total_ms, uint32_t *launches);
 

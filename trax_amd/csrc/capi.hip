@@ -307,7 +307,7 @@ extern "C" int trax_ctx_destroy(trax_ctx* ctx) {
 extern "C" int trax_ctx_kernel_timer(trax_ctx* ctx, int kernel) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
-  if (kernel < 0 || kernel > TRAX_KERNEL_TRIPLET_PLACE) return fail(ctx, TRAX_ERR_INVALID, "unknown kernel id");
+  if (kernel < 0 || kernel > TRAX_KERNEL_ACC_CUT) return fail(ctx, TRAX_ERR_INVALID, "unknown kernel id");
   if (ctx->ev_start.empty()) {
     const size_t n = 256;
     for (size_t i = 0; i < n; ++i) {
@@ -813,6 +813,8 @@ extern "C" int trax_triplet_predict_filter(trax_ctx* ctx, const trax_seed_config
   TRAX_TIMED(ctx, TRAX_KERNEL_PREDICT_FILTER, s,
              trax::launch_predict_filter_walk(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
                                               ws->cand_counts, tiles, slab, acc, s, ctx->fused_variant));
+  TRAX_TIMED(ctx, TRAX_KERNEL_ACC_CUT, s,
+             trax::launch_acc_cut(sp, h, out->totals + 0, pc, slab, acc, s));
   TRAX_HIP_CHECK(ctx, trax::launch_acc_redo(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc,
                                             ws->cand_counts, slab, acc, s));
   TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s,

@@ -468,19 +468,24 @@ hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const 
 // order and walk order, and the candidate records need not be written.
 // Pass 1 walks each tile of 64 pairs as k_predict_walk does (the count
 // budget and the store-window hits, in LDS), queues the tile's
-// buffered candidates densely in the wave's LDS and runs the cuts over the
-// queue, each cut on the previous one's survivors.  The tile's accepted
-// (pair, third hit) records go to its slab, or, when more than kAccSlots
-// are accepted, the per-pair accepted counts.  The candidate budgets still
-// give the candidate offsets and total.  After the prefix sum of the
-// accepted tile totals, pass 2 writes the triplets.
-// A tile's slab holds its accepted records as (pair, hit1, hit2, hit3), so
-// that placing the triplets needs no pair-record gather.
+// buffered candidates densely in the wave's LDS and runs the phi cut over
+// the queue.  The tile's phi survivors go to its slab as (lane | layer
+// triplet << 6, hit1, hit2, hit3) records in pair and slot order, their
+// number to the tile's accepted total (more than kAccSlots: the tile is
+// left to k_acc_redo, which walks it again with every cut and writes the
+// accepted records, or the per-pair accepted counts when there are more
+// than kAccSlots).  The theta and circle cuts then run over the survivors
+// of 64 tiles at a time (k_acc_cut), on full waves: a tile has ~14 phi
+// survivors and ~7 theta survivors on PU200, so in the walk each of those
+// cuts was a wave pass of mostly idle lanes (1.21 ms for the walk with all
+// three cuts, 1.06 ms with the phi cut, plus 0.09 ms for k_acc_cut).
+// k_acc_cut leaves each tile's accepted records in its slab as (pair, hit1,
+// hit2, hit3), in order, and their number as the tile's accepted total.
+// The candidate budgets still give the candidate offsets and total.  After
+// the prefix sum of the accepted tile totals, pass 2 writes the triplets.
 constexpr uint32_t kAccSlots = kTileSlots / 4;  // uint4 records per tile slab
 constexpr uint32_t kTileRedo = 0xFFFFFFFFu;     // accepted total not known yet
 
-// Accepted store-window hits of pair `it` among its slots [skip, budget),
-// in walk order (slots past the store window's hits hold no hit).
 // Accepted store-window hits of pair `it` among its slots [skip, budget),
 // in walk order (slots past the store window's hits hold no hit).  The rare
 // path: the same visit order as predict_walk, one hit at a time, so the cuts
@@ -564,9 +569,9 @@ struct FuseLds {
 };
 
 // One tile of pass 1 for the wave: pair q = t * 64 + lane.  One walk fills
-// the buffers and the slot budgets, the cuts run over the dense queue of
-// filled slots, and the tile's accepted total and slab records (or per-pair
-// counts when more than kAccSlots are accepted) are written.  Returns false,
+// the buffers and the slot budgets, the phi cut runs over the dense queue of
+// filled slots, and the tile's survivor total and survivor records are
+// written (kTileRedo when more than kAccSlots survive).  Returns false,
 // leaving the tile to k_acc_redo, when the spill pool overflows.
 template <uint32_t BUF>
 __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
@@ -615,26 +620,9 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   }
   L.acc[lane] = 0u;
   wave_lds_sync();
-  // entry -> (owner lane, third hit); the pair's hits travel from the owner
-  // lane by shuffles
-  auto load = [&](uint32_t ent, bool live, float* v, uint32_t* i3) {
-    const bool spilled = ent >= kSpillTag;
-    const uint32_t tag = spilled ? (uint32_t)L.spill_tag[ent - kSpillTag] : ent;
-    const uint32_t l = tag & 63u;
-    v[0] = __shfl(it.p1x, (int)l, 64);
-    v[1] = __shfl(it.p1y, (int)l, 64);
-    v[2] = __shfl(it.p1z, (int)l, 64);
-    v[3] = __shfl(it.p2x, (int)l, 64);
-    v[4] = __shfl(it.p2y, (int)l, 64);
-    v[5] = __shfl(it.p2z, (int)l, 64);
-    const uint32_t lt = (uint32_t)__shfl(it.lt, (int)l, 64);
-    if (live) *i3 = spilled ? L.spill_hit[ent - kSpillTag] : L.buf[(tag >> 6) * 64u + l];
-    return lt;
-  };
-  auto none = [](uint32_t) {};
   // the phi cut's pair angle, once per pair
   const float a1 = nq ? trax_atan2f((it.p2y - it.p1y), (it.p2x - it.p1x)) : 0.0f;
-  const uint32_t n1 = queue_cut<true>(L.queue, nq, [&](uint32_t ent, bool live) {
+  queue_cut<false>(L.queue, nq, [&](uint32_t ent, bool live) {
     const bool spilled = ent >= kSpillTag;
     const uint32_t tag = spilled ? (uint32_t)L.spill_tag[ent - kSpillTag] : ent;
     const uint32_t l = tag & 63u;
@@ -644,18 +632,6 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
     if (!live) return false;
     const uint32_t i3 = spilled ? L.spill_hit[ent - kSpillTag] : L.buf[(tag >> 6) * 64u + l];
     return filter_phi_a1(sp, lt, ang, x2, y2, h.x[i3], h.y[i3]);
-  }, none);
-  const uint32_t n2 = queue_cut<true>(L.queue, n1, [&](uint32_t ent, bool live) {
-    float v[6];
-    uint32_t i3 = 0;
-    const uint32_t lt = load(ent, live, v, &i3);
-    return live && filter_theta(sp, lt, v[0], v[1], v[2], v[3], v[4], v[5], h.x[i3], h.y[i3], h.z[i3]);
-  }, none);
-  queue_cut<false>(L.queue, n2, [&](uint32_t ent, bool live) {
-    float v[6];
-    uint32_t i3 = 0;
-    const uint32_t lt = load(ent, live, v, &i3);
-    return live && filter_circle(sp, lt, v[0], v[1], v[3], v[4], h.x[i3], h.y[i3]);
   }, [&](uint32_t ent) {
     if (ent >= kSpillTag)
       L.spill_acc[ent - kSpillTag] = 1;
@@ -668,28 +644,26 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
     a += (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) ? 1u : 0u;
   const uint32_t aincl = wave_inclusive_scan(a);
   const uint32_t A = __shfl(aincl, 63, 64);
-  if (lane == 0) acc_tiles[t] = A;
+  if (lane == 0) acc_tiles[t] = A <= kAccSlots ? A : kTileRedo;
   uint4* slab = acc_slab + (uint64_t)t * kAccSlots;
-  if (A <= kAccSlots) {
+  if (A <= kAccSlots && a) {
     uint32_t pos = aincl - a;
-    if (a) {
-      const uint2 pr = pairs[q];  // read again: keeping it through the walk costs registers
-      for (uint32_t m = mask; m; m &= m - 1u)
-        slab[pos++] = make_uint4(q, pr.x, pr.y, s_col[(uint32_t)__builtin_ctz(m) * 64u]);
-      for (uint32_t k = 0; k < ns; ++k)
-        if (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) slab[pos++] = make_uint4(q, pr.x, pr.y, L.spill_hit[k]);
-    }
-  } else {
-    reinterpret_cast<uint32_t*>(slab)[lane] = a;  // pass 2 walks this tile's pairs again
+    const uint2 pr = pairs[q];  // read again: keeping it through the walk costs registers
+    const uint32_t tag = lane | (it.lt << 6);
+    for (uint32_t m = mask; m; m &= m - 1u)
+      slab[pos++] = make_uint4(tag, pr.x, pr.y, s_col[(uint32_t)__builtin_ctz(m) * 64u]);
+    for (uint32_t k = 0; k < ns; ++k)
+      if (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) slab[pos++] = make_uint4(tag, pr.x, pr.y, L.spill_hit[k]);
   }
   wave_lds_sync();  // the buffers, queue and pool are reused by the wave's next tile
   return true;
 }
 
 // Default: an 8-deep lane buffer, four waves per block, seven waves per
-// SIMD (at most 72 VGPRs, under 22.8 KB of LDS per block); PU200: 1.22 ms
-// against 1.31 ms at the six the unconstrained register allocation (79
-// VGPRs) allows, 1.26 ms at eight.
+// SIMD (at most 72 VGPRs, under 22.8 KB of LDS per block).  With all three
+// cuts in the walk this shape ran PU200 in 1.22 ms against 1.31 ms at the
+// six waves the unconstrained register allocation (79 VGPRs) allowed and
+// 1.26 ms at eight.
 template <uint32_t BUF, uint32_t NW, int WPE>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) void k_predict_filter_walk(SeedParams sp, SortedHits h,
                                                              const uint32_t* __restrict__ grid,
@@ -725,10 +699,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
   }
 }
 
-// Tiles whose spill pool overflowed (more than kSpill slots past the lane
-// buffers; none in the PU200 and ttbar workloads): each lane walks its
-// pair's slots one by one with the cuts, once to count and once to write the
-// records (or the per-pair counts).  One lane per tile finds the marked
+// Tiles the walk marked kTileRedo (spill pool overflowed: more than kSpill
+// slots past the lane buffers; or more than kAccSlots phi survivors; none in
+// the PU200 and ttbar workloads): each lane walks its pair's slots one by one
+// with all three cuts, once to count and once to write the records (or the
+// per-pair counts).  It runs after k_acc_cut, which leaves these tiles alone.  One lane per tile finds the marked
 // tiles; the wave then takes them in turn.
 __global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, const uint32_t* __restrict__ grid,
                                                  const uint2* __restrict__ pairs,
@@ -767,6 +742,123 @@ __global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, co
         reinterpret_cast<uint32_t*>(slab)[lane] = a;
       }
       if (lane == 0) acc_tiles[t] = A;
+    }
+  }
+}
+
+// Theta and circle cuts over the phi survivors of kCutTiles consecutive
+// tiles per block and round.  The tiles' survivor totals (written by the
+// walk) are scanned in wave 0.  The survivors are then taken as one flat
+// range, in windows of whole tiles holding at most 256 * kCutPer records:
+// each thread reads kCutPer records (record r belongs to the tile whose
+// running total first exceeds r) and evaluates the cuts, the accept bits go
+// to an LDS bitmap, and after a barrier every accepted record is written back
+// to its tile's slab as (pair, hit1, hit2, hit3) at the number of accepted
+// records before it in the tile (all of the window's records are read before
+// any is overwritten, and a record only moves down), with the tiles'
+// accepted totals.  Tiles the walk marked kTileRedo are left to k_acc_redo.
+constexpr uint32_t kCutTiles = 64;
+constexpr uint32_t kCutPer = 4;
+constexpr uint32_t kCutWindow = 256u * kCutPer;
+
+// Set bits of bitmap w in [lo, hi).
+__device__ __forceinline__ uint32_t bits_in(const uint32_t* w, uint32_t lo, uint32_t hi) {
+  uint32_t n = 0;
+  for (uint32_t k = lo >> 5; k < ((hi + 31u) >> 5); ++k) {
+    uint32_t m = w[k];
+    const uint32_t b0 = k << 5;
+    if (lo > b0) m &= ~0u << (lo - b0);
+    if (hi < b0 + 32u) m &= (1u << (hi - b0)) - 1u;
+    n += (uint32_t)__popc(m);
+  }
+  return n;
+}
+
+__global__ __launch_bounds__(256) void k_acc_cut(SeedParams sp, SortedHits h, const uint32_t* __restrict__ d_npairs,
+                                                 uint64_t pair_capacity, uint4* __restrict__ acc_slab,
+                                                 uint32_t* __restrict__ acc_tiles) {
+  __shared__ uint32_t s_incl[kCutTiles];
+  __shared__ uint32_t s_ok[kCutWindow / 32u];
+  __shared__ unsigned long long s_redo;
+  static_assert(kAccSlots <= kCutWindow, "a window holds a whole tile");
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, tid = threadIdx.x;
+  const uint64_t nP64 = *d_npairs;
+  const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
+  const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
+  for (uint32_t t0 = blockIdx.x * kCutTiles; t0 < ntiles; t0 += gridDim.x * kCutTiles) {
+    if (wave == 0) {
+      const uint32_t t = t0 + lane;
+      const uint32_t v = t < ntiles ? acc_tiles[t] : 0u;
+      const bool redo = v == kTileRedo;
+      const uint32_t incl = wave_inclusive_scan(redo ? 0u : v);
+      s_incl[lane] = incl;
+      const uint64_t m = __ballot(redo);
+      if (lane == 0) s_redo = m;
+    }
+    __syncthreads();
+    const uint32_t nt = ntiles - t0 < kCutTiles ? ntiles - t0 : kCutTiles;
+    for (uint32_t tb = 0; tb < nt;) {
+      // window: tiles [tb, te] whose records fit after R0
+      const uint32_t R0 = tb ? s_incl[tb - 1u] : 0u;
+      const uint64_t fit = __ballot(lane < nt && s_incl[lane] <= R0 + kCutWindow);
+      const uint32_t te = 63u - (uint32_t)__builtin_clzll(fit);  // >= tb: a tile holds at most kAccSlots
+      const uint32_t R1 = s_incl[te];
+      uint4 rec[kCutPer];
+      uint32_t lo[kCutPer];
+      bool ok[kCutPer];
+#pragma unroll
+      for (uint32_t u = 0; u < kCutPer; ++u) {
+        const uint32_t r = R0 + u * 256u + tid;
+        rec[u] = make_uint4(0u, 0u, 0u, 0u);
+        lo[u] = 0u;
+        if (r < R1) {
+          uint32_t a = tb, z = te;  // first tile whose running total exceeds r
+          while (a < z) {
+            const uint32_t mid = (a + z) >> 1;
+            if (s_incl[mid] > r) z = mid; else a = mid + 1u;
+          }
+          lo[u] = a;
+          rec[u] = acc_slab[(uint64_t)(t0 + a) * kAccSlots + (r - (a ? s_incl[a - 1u] : 0u))];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kCutPer; ++u) {
+        const uint32_t r = R0 + u * 256u + tid;
+        ok[u] = false;
+        if (r < R1) {
+          const uint32_t lt = rec[u].x >> 6;
+          const float x1 = h.x[rec[u].y], y1 = h.y[rec[u].y], z1 = h.z[rec[u].y];
+          const float x2 = h.x[rec[u].z], y2 = h.y[rec[u].z], z2 = h.z[rec[u].z];
+          const float x3 = h.x[rec[u].w], y3 = h.y[rec[u].w], z3 = h.z[rec[u].w];
+          ok[u] = filter_theta(sp, lt, x1, y1, z1, x2, y2, z2, x3, y3, z3) &&
+                  filter_circle(sp, lt, x1, y1, x2, y2, x3, y3);
+        }
+        const uint64_t m = __ballot(ok[u]);
+        if (lane == 0) {
+          s_ok[u * 8u + 2u * wave] = (uint32_t)m;
+          s_ok[u * 8u + 2u * wave + 1u] = (uint32_t)(m >> 32);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (uint32_t u = 0; u < kCutPer; ++u) {
+        if (!ok[u]) continue;
+        const uint32_t r = R0 + u * 256u + tid;
+        const uint32_t a = lo[u];
+        const uint32_t tb0 = (a ? s_incl[a - 1u] : 0u) - R0;  // the tile's first record in the window
+        const uint32_t dst = bits_in(s_ok, tb0, r - R0);
+        acc_slab[(uint64_t)(t0 + a) * kAccSlots + dst] =
+            make_uint4((t0 + a) * kPredTile + (rec[u].x & 63u), rec[u].y, rec[u].z, rec[u].w);
+      }
+      if (tid <= te - tb) {
+        const uint32_t a = tb + tid;
+        if (!((s_redo >> a) & 1ull)) {
+          const uint32_t f = (a ? s_incl[a - 1u] : 0u) - R0;
+          acc_tiles[t0 + a] = bits_in(s_ok, f, s_incl[a] - R0);
+        }
+      }
+      __syncthreads();  // the bitmap is rewritten for the next window
+      tb = te + 1u;
     }
   }
 }
@@ -938,6 +1030,17 @@ hipError_t launch_acc_redo(const SeedParams& sp, const SortedHits& h, const uint
   const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 4096u ? 4096u : b));
   hipLaunchKernelGGL(k_acc_redo, dim3(blocks), dim3(64), 0, stream, sp, h, grid, pairs, pair_off, d_npairs,
                      pair_capacity, counts, acc_slab, acc_tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_acc_cut(const SeedParams& sp, const SortedHits& h, const uint32_t* d_npairs,
+                          uint64_t pair_capacity, uint4* acc_slab, uint32_t* acc_tiles, hipStream_t stream) {
+  // kCutTiles tiles per block and round
+  const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
+  uint64_t b = (tiles + kCutTiles - 1u) / kCutTiles;
+  const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 16384u ? 16384u : b));
+  hipLaunchKernelGGL(k_acc_cut, dim3(blocks), dim3(256), 0, stream, sp, h, d_npairs, pair_capacity, acc_slab,
+                     acc_tiles);
   return hipGetLastError();
 }
 

@@ -47,6 +47,7 @@ SHORT = [
     ("k_predict_walk", "predict_walk"),
     ("k_predict_filter_walk", "predict_filter"),
     ("k_acc_redo", "acc_redo"),
+    ("k_acc_cut", "acc_cut"),
     ("k_acc_place", "triplet_place"),
     ("k_acc_offsets", "acc_offsets"),
     ("k_predict_place", "predict_place"),

@@ -20,7 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libtraxhip.so")
 # Kernel ids of the context's kernel timer (TRAX_KERNEL_* in include/trax_hip.h).
 KERNELS = {"grid_build": 1, "pairs": 2, "predict_walk": 3, "predict_place": 4, "filter_mask": 5,
-           "filter_store": 6, "scan": 7, "connect": 8, "predict_filter": 9, "triplet_place": 10}
+           "filter_store": 6, "scan": 7, "connect": 8, "predict_filter": 9, "triplet_place": 10,
+           "acc_cut": 11}
 # Store-window hits per pair the single-call predictor buffers before a pair walks again.
 PREDICT_BUFFER = 16
 PREDICT_TILE, PREDICT_TILE_SLOTS = 64, 512  # TRAX_PREDICT_TILE, TRAX_PREDICT_TILE_SLOTS
