@@ -78,8 +78,9 @@ def time_events(fn, reps: int, torch):
 # write at least once (inputs read once, outputs written once), from the
 # batch's counts.  c: hits, pairs, cands (candidate slots), triplets, cells.
 ALGO_BYTES = {
-    # 7 input columns read, 9 sorted columns written, one uint32 per cell
-    "grid_build": lambda c: 64 * c["hits"] + 4 * c["cells"],
+    # 7 input columns read (28 B), 9 sorted columns (36 B), the (z, phi) pair
+    # (8 B) and the (x, y, z, phi) quadruple (16 B) written, one uint32 per cell
+    "grid_build": lambda c: 88 * c["hits"] + 4 * c["cells"],
     # x, y, z, phi of every hit and the cells read, one uint2 per pair written
     # (the stage's count and store launches share these bytes)
     "pairs": lambda c: 16 * c["hits"] + 4 * c["cells"] + 8 * c["pairs"],

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TRAX_ABI_VERSION 7
+#define TRAX_ABI_VERSION 8
 
 #define TRAX_OK 0
 #define TRAX_ERR_INVALID (-1)  /* bad argument or configuration */
@@ -84,14 +84,17 @@ typedef struct {
 #define TRAX_PAIRS_SECTOR 1u
 
 /* Hit columns.  Input: x..evt.  Grid-sorted output: all of them, plus
- * orig_index (input index of each sorted hit), phi (atan2(y, x)) and zphi
- * (2 floats per hit: z, phi interleaved, read by the window walks). */
+ * orig_index (input index of each sorted hit), phi (atan2(y, x)), zphi
+ * (2 floats per hit: z, phi interleaved, read by the window walks) and
+ * xyzphi (4 floats per hit: x, y, z, phi, 16-byte aligned, read by the pair,
+ * prediction and cut kernels as one load per hit). */
 typedef struct {
   float *x, *y, *z;
   uint32_t *layer, *det_id, *hit_id, *evt;
   uint32_t *orig_index;
   float *phi;
   float *zphi;
+  float *xyzphi;
 } trax_hits;
 
 /* Event layout, device arrays (EventSupplement.h / LayerSupplement.h). */

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_only_entry_points():
     lib = runtime.lib()
-    assert lib.trax_abi_version() == 7
+    assert lib.trax_abi_version() == 8
     # connectivity table: a power of two >= 2 * n_base slots (at least 64), plus one, 16 B each
     for n, slots in ((0, 64), (1, 64), (32, 64), (33, 128), (1000, 2048), (1 << 20, 1 << 21)):
         assert lib.trax_connectivity_workspace_bytes(n) == (slots + 1) * 16

@@ -72,6 +72,9 @@ def _grid_parity(ctx, sc, batch):
     zphi = s["zphi"][:2 * n].cpu().numpy().reshape(-1, 2)
     assert np.array_equal(zphi[:, 0].view(np.uint32), o["z"].view(np.uint32))
     assert np.array_equal(zphi[:, 1].view(np.uint32), phi.view(np.uint32))
+    xyzphi = s["xyzphi"][:4 * n].cpu().numpy().reshape(-1, 4)
+    for c, want in enumerate((o["x"], o["y"], o["z"], phi)):
+        assert np.array_equal(xyzphi[:, c].view(np.uint32), want.view(np.uint32)), ("xyzphi", c)
     return o
 
 

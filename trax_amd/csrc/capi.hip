@@ -181,10 +181,14 @@ trax::SortedHits sorted_view(const trax_hits* h) {
   s.layer = h->layer;
   s.evt = h->evt;
   s.zphi = reinterpret_cast<const float2*>(h->zphi);
+  s.xyzphi = reinterpret_cast<const float4*>(h->xyzphi);
   return s;
 }
 
-bool sorted_ok(const trax_hits* h) { return h && h->x && h->y && h->z && h->phi && h->zphi; }
+bool sorted_ok(const trax_hits* h) {
+  return h && h->x && h->y && h->z && h->phi && h->zphi && h->xyzphi &&
+         !(reinterpret_cast<uintptr_t>(h->xyzphi) & 15u);
+}
 
 int check_ctx(trax_ctx* ctx) {
   if (!ctx) return TRAX_ERR_INVALID;
@@ -495,7 +499,8 @@ extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const
   if (!in || !in->x || !in->y || !in->z || !in->layer || !in->det_id || !in->hit_id || !in->evt)
     return fail(ctx, TRAX_ERR_INVALID, "incomplete input hits");
   if (!sorted || !sorted->x || !sorted->y || !sorted->z || !sorted->layer || !sorted->det_id ||
-      !sorted->hit_id || !sorted->evt || !sorted->orig_index || !sorted->phi || !sorted->zphi)
+      !sorted->hit_id || !sorted->evt || !sorted->orig_index || !sorted->phi || !sorted->zphi ||
+      !sorted->xyzphi || (reinterpret_cast<uintptr_t>(sorted->xyzphi) & 15u))
     return fail(ctx, TRAX_ERR_INVALID, "incomplete sorted-hit buffers");
   if (!d_grid || !layout->event_offset || !layout->layer_nhits || !layout->layer_offset)
     return fail(ctx, TRAX_ERR_INVALID, "null grid or layout array");
@@ -507,6 +512,7 @@ extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const
   gl.olayer = sorted->layer; gl.odet = sorted->det_id; gl.ohid = sorted->hit_id;
   gl.oevt = sorted->evt; gl.oorig = sorted->orig_index; gl.ophi = sorted->phi;
   gl.ozphi = reinterpret_cast<float2*>(sorted->zphi);
+  gl.oxyzphi = reinterpret_cast<float4*>(sorted->xyzphi);
   gl.event_offset = layout->event_offset;
   gl.layer_nhits = layout->layer_nhits;
   gl.layer_offset = layout->layer_offset;

@@ -49,12 +49,18 @@ struct SortedHits {
   const uint32_t* __restrict__ layer;
   const uint32_t* __restrict__ evt;
   const float2* __restrict__ zphi;  // (z, phi) per hit, for the window walks
+  const float4* __restrict__ xyzphi;  // (x, y, z, phi) per hit, one 16-byte load
 };
 
 // (z, phi) of sorted hit idx, addressed with a 32-bit byte offset (the C ABI
 // caps hit counts below 2^29) so the load takes the scalar-base form.
 __device__ __forceinline__ float2 load_zphi(const float2* __restrict__ zphi, uint32_t idx) {
   return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(zphi) + (idx << 3));
+}
+
+// (x, y, z, phi) of sorted hit idx, the same addressing.
+__device__ __forceinline__ float4 load_xyzphi(const float4* __restrict__ xyzphi, uint32_t idx) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(xyzphi) + (idx << 4));
 }
 
 // Decoupled look-back control block shared by the scan and the filter

@@ -93,7 +93,9 @@ __global__ __launch_bounds__(256) void k_filter_mask(SeedParams sp, SortedHits h
         if (cd.x != TRAX_UNWRITTEN) {
           const uint2 pr = pairs[cd.x];
           const uint32_t lt = segment_of(offs, K + 1, c) % P;
-          keep = filter_phi(sp, lt, h.x[pr.x], h.y[pr.x], h.x[pr.y], h.y[pr.y], h.x[cd.y], h.y[cd.y]);
+          const float4 p1 = load_xyzphi(h.xyzphi, pr.x), p2 = load_xyzphi(h.xyzphi, pr.y),
+                       p3 = load_xyzphi(h.xyzphi, cd.y);
+          keep = filter_phi(sp, lt, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
         }
       }
       const uint64_t m = __ballot(keep);
@@ -107,8 +109,9 @@ __global__ __launch_bounds__(256) void k_filter_mask(SeedParams sp, SortedHits h
       const uint2 cd = cands[c];
       const uint2 pr = pairs[cd.x];
       const uint32_t lt = segment_of(offs, K + 1, c) % P;
-      return filter_theta(sp, lt, h.x[pr.x], h.y[pr.x], h.z[pr.x], h.x[pr.y], h.y[pr.y], h.z[pr.y],
-                          h.x[cd.y], h.y[cd.y], h.z[cd.y]);
+      const float4 p1 = load_xyzphi(h.xyzphi, pr.x), p2 = load_xyzphi(h.xyzphi, pr.y),
+                   p3 = load_xyzphi(h.xyzphi, cd.y);
+      return filter_theta(sp, lt, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z);
     });
     // stage 3: circle fit on the survivors of both angle cuts
     for (uint32_t j = lane; j < nq; j += 64u) {
@@ -117,8 +120,9 @@ __global__ __launch_bounds__(256) void k_filter_mask(SeedParams sp, SortedHits h
       const uint2 cd = cands[c];
       const uint2 pr = pairs[cd.x];
       const uint32_t lt = segment_of(offs, K + 1, c) % P;
-      pass[e] = filter_circle(sp, lt, h.x[pr.x], h.y[pr.x], h.x[pr.y], h.y[pr.y], h.x[cd.y],
-                              h.y[cd.y]) ? 1 : 0;
+      const float4 p1 = load_xyzphi(h.xyzphi, pr.x), p2 = load_xyzphi(h.xyzphi, pr.y),
+                   p3 = load_xyzphi(h.xyzphi, cd.y);
+      pass[e] = filter_circle(sp, lt, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y) ? 1 : 0;
     }
     wave_lds_sync();
 #pragma unroll

@@ -60,6 +60,7 @@ struct GridArgs {
   uint32_t* oorig;
   float* ophi;
   float2* ozphi;
+  float4* oxyzphi;
   const uint32_t* __restrict__ event_offset;
   const uint32_t* __restrict__ layer_nhits;
   const uint32_t* __restrict__ layer_offset;
@@ -493,6 +494,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
               a.oz[dst] = __builtin_bit_cast(float, va);
               a.ophi[dst] = __builtin_bit_cast(float, vb);
               a.ozphi[dst] = make_float2(__builtin_bit_cast(float, va), __builtin_bit_cast(float, vb));
+              // x and y of this sorted position were stored by this thread in round 0
+              a.oxyzphi[dst] = make_float4(a.ox[dst], a.oy[dst], __builtin_bit_cast(float, va),
+                                           __builtin_bit_cast(float, vb));
             } else if (r == 2) {
               a.olayer[dst] = va;
               a.odet[dst] = vb;
@@ -565,6 +569,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
         a.oorig[dst] = src;
         a.ophi[dst] = phi;
         a.ozphi[dst] = make_float2(z, phi);
+        a.oxyzphi[dst] = make_float4(x, y, z, phi);
       }
       __syncthreads();
     }
@@ -600,7 +605,7 @@ hipError_t launch_grid_build(const SeedParams& sp, const GridLaunch& gl, hipStre
   a.ilayer = gl.ilayer; a.idet = gl.idet; a.ihid = gl.ihid; a.ievt = gl.ievt;
   a.ox = gl.ox; a.oy = gl.oy; a.oz = gl.oz;
   a.olayer = gl.olayer; a.odet = gl.odet; a.ohid = gl.ohid; a.oevt = gl.oevt;
-  a.oorig = gl.oorig; a.ophi = gl.ophi; a.ozphi = gl.ozphi;
+  a.oorig = gl.oorig; a.ophi = gl.ophi; a.ozphi = gl.ozphi; a.oxyzphi = gl.oxyzphi;
   a.event_offset = gl.event_offset; a.layer_nhits = gl.layer_nhits; a.layer_offset = gl.layer_offset;
   a.n_hits = gl.n_hits;
   a.grid = gl.grid;

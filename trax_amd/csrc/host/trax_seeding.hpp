@@ -207,19 +207,19 @@ class HitCollection {
     dx.upload(x, s), dy.upload(y, s), dz.upload(z, s);
     dlayer.upload(layer, s), ddet.upload(detId, s), dhid.upload(hitId, s), devt.upload(evt, s);
     const size_t n = size();
-    sx.resize(n), sy.resize(n), sz.resize(n), sphi.resize(n), szphi.resize(2 * n);
+    sx.resize(n), sy.resize(n), sz.resize(n), sphi.resize(n), szphi.resize(2 * n), sxyzphi.resize(4 * n);
     slayer.resize(n), sdet.resize(n), shid.resize(n), sevt.resize(n), sorig.resize(n);
   }
   trax_hits input() const {
     return {dx.get(), dy.get(), dz.get(), dlayer.get(), ddet.get(), dhid.get(), devt.get(), nullptr, nullptr,
-            nullptr};
+            nullptr, nullptr};
   }
   trax_hits sorted() const {
     return {sx.get(), sy.get(), sz.get(), slayer.get(), sdet.get(), shid.get(), sevt.get(), sorig.get(), sphi.get(),
-            szphi.get()};
+            szphi.get(), sxyzphi.get()};
   }
   // device columns: unsorted input and grid-sorted output
-  DeviceArray<float> dx, dy, dz, sx, sy, sz, sphi, szphi;
+  DeviceArray<float> dx, dy, dz, sx, sy, sz, sphi, szphi, sxyzphi;
   DeviceArray<uint32_t> dlayer, ddet, dhid, devt, slayer, sdet, shid, sevt, sorig;
 };
 

@@ -43,6 +43,7 @@ struct GridLaunch {
   uint32_t *olayer, *odet, *ohid, *oevt, *oorig;
   float* ophi;
   float2* ozphi;
+  float4* oxyzphi;
   const uint32_t *event_offset, *layer_nhits, *layer_offset;
   uint32_t n_hits;
   uint32_t* grid;

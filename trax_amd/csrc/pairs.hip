@@ -184,7 +184,8 @@ __device__ __forceinline__ PairItem pair_item(const SeedParams& sp, const Sorted
   const uint32_t l1 = sp.layer1[it.p] - 1;
   it.g1 = grid + ((uint64_t)e * L + l1) * C;
   it.i = g2[0] + (w - offs[it.k]);
-  it.win = pair_window(sp, it.p, h.x[it.i], h.y[it.i], h.z[it.i], h.phi[it.i], sp.min_radius[l1],
+  const float4 o = load_xyzphi(h.xyzphi, it.i);
+  it.win = pair_window(sp, it.p, o.x, o.y, o.z, o.w, sp.min_radius[l1],
                        sp.max_radius[l1], sp.acos_min_r[l1], sp.acos_max_r[l1]);
   return it;
 }

@@ -135,8 +135,9 @@ __device__ __forceinline__ PredItem pred_item(const SeedParams& sp, const Sorted
   const uint2 pr = pairs[q];
   const uint32_t layer = sp.layer3[it.lt] - 1;
   it.g3 = grid + ((uint64_t)e * L + layer) * C;
-  it.p1x = h.x[pr.x], it.p1y = h.y[pr.x], it.p1z = h.z[pr.x], it.phi1 = h.phi[pr.x];
-  it.p2x = h.x[pr.y], it.p2y = h.y[pr.y], it.p2z = h.z[pr.y], it.phi2 = h.phi[pr.y];
+  const float4 a = load_xyzphi(h.xyzphi, pr.x), b = load_xyzphi(h.xyzphi, pr.y);
+  it.p1x = a.x, it.p1y = a.y, it.p1z = a.z, it.phi1 = a.w;
+  it.p2x = b.x, it.p2y = b.y, it.p2z = b.z, it.phi2 = b.w;
   it.minR = sp.min_radius[layer];
   it.maxR = sp.max_radius[layer];
   it.acosMinR = sp.acos_min_r[layer];
@@ -514,9 +515,10 @@ __device__ __forceinline__ void accept_walk(const SeedParams& sp, const SortedHi
                          (win.wrap && ((win.phiLow <= actPhi && actPhi <= PI) ||
                                        (-PI <= actPhi && actPhi <= win.phiHigh))));
         if (!in) continue;
-        if (seen >= skip && filter_accept(sp, it.lt, it.p1x, it.p1y, it.p1z, it.p2x, it.p2y, it.p2z, h.x[idx],
-                                          h.y[idx], h.z[idx]))
-          emit(idx);
+        if (seen >= skip) {
+          const float4 c = load_xyzphi(h.xyzphi, idx);
+          if (filter_accept(sp, it.lt, it.p1x, it.p1y, it.p1z, it.p2x, it.p2y, it.p2z, c.x, c.y, c.z)) emit(idx);
+        }
         if (++seen >= budget) return;
       }
     }
@@ -631,7 +633,8 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
     const uint32_t lt = (uint32_t)__shfl(it.lt, (int)l, 64);
     if (!live) return false;
     const uint32_t i3 = spilled ? L.spill_hit[ent - kSpillTag] : L.buf[(tag >> 6) * 64u + l];
-    return filter_phi_a1(sp, lt, ang, x2, y2, h.x[i3], h.y[i3]);
+    const float4 c = load_xyzphi(h.xyzphi, i3);
+    return filter_phi_a1(sp, lt, ang, x2, y2, c.x, c.y);
   }, [&](uint32_t ent) {
     if (ent >= kSpillTag)
       L.spill_acc[ent - kSpillTag] = 1;
@@ -827,11 +830,10 @@ __global__ __launch_bounds__(256) void k_acc_cut(SeedParams sp, SortedHits h, co
         ok[u] = false;
         if (r < R1) {
           const uint32_t lt = rec[u].x >> 6;
-          const float x1 = h.x[rec[u].y], y1 = h.y[rec[u].y], z1 = h.z[rec[u].y];
-          const float x2 = h.x[rec[u].z], y2 = h.y[rec[u].z], z2 = h.z[rec[u].z];
-          const float x3 = h.x[rec[u].w], y3 = h.y[rec[u].w], z3 = h.z[rec[u].w];
-          ok[u] = filter_theta(sp, lt, x1, y1, z1, x2, y2, z2, x3, y3, z3) &&
-                  filter_circle(sp, lt, x1, y1, x2, y2, x3, y3);
+          const float4 p1 = load_xyzphi(h.xyzphi, rec[u].y), p2 = load_xyzphi(h.xyzphi, rec[u].z),
+                       p3 = load_xyzphi(h.xyzphi, rec[u].w);
+          ok[u] = filter_theta(sp, lt, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z) &&
+                  filter_circle(sp, lt, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
         }
         const uint64_t m = __ballot(ok[u]);
         if (lane == 0) {

@@ -58,7 +58,8 @@ PAIR_GENERATORS = {"beamspot": 0, "sector": 1}  # TRAX_PAIRS_BEAMSPOT / TRAX_PAI
 
 class HitsC(C.Structure):
     _fields_ = [("x", _vp), ("y", _vp), ("z", _vp), ("layer", _vp), ("det_id", _vp),
-                ("hit_id", _vp), ("evt", _vp), ("orig_index", _vp), ("phi", _vp), ("zphi", _vp)]
+                ("hit_id", _vp), ("evt", _vp), ("orig_index", _vp), ("phi", _vp), ("zphi", _vp),
+                ("xyzphi", _vp)]
 
 
 class LayoutC(C.Structure):
@@ -352,6 +353,7 @@ class ChainBuffers:
         self.sorted = {k: _empty(n, "f" if k in ("x", "y", "z", "phi") else "u", device)
                        for k in ("x", "y", "z", "layer", "det_id", "hit_id", "evt", "orig_index", "phi")}
         self.sorted["zphi"] = _empty(2 * n, "f", device)
+        self.sorted["xyzphi"] = _empty(4 * n, "f", device)
         K1 = self.max_events * len(sc.triplets) + 1
         self.grid = _empty(self.max_events * sc.n_layers * sc.cells_per_layer, "u", device)
         L = lib()
@@ -373,7 +375,7 @@ class ChainBuffers:
     def sorted_c(self) -> HitsC:
         s = self.sorted
         return HitsC(*[_ptr(s[k]) for k in ("x", "y", "z", "layer", "det_id", "hit_id", "evt",
-                                            "orig_index", "phi", "zphi")])
+                                            "orig_index", "phi", "zphi", "xyzphi")])
 
     def outputs_c(self) -> OutputsC:
         return OutputsC(_ptr(self.pairs), self.pair_capacity, _ptr(self.pair_offsets),
