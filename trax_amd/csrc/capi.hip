@@ -826,13 +826,12 @@ extern "C" int trax_triplet_predict_filter(trax_ctx* ctx, const trax_seed_config
   TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s,
              trax::launch_scan_u32(tiles, tiles, tiles_max + 1, tiles + tiles_max + 1, 1, ctx->ctrl,
                                    ctx->status, s, scan_sub(ctx)));
-  TRAX_HIP_CHECK(ctx, trax::launch_pred_offsets(tiles, ws->cand_counts, out->pair_offsets, out->totals + 0, pc,
-                                                K + 1, out->cand_offsets, out->totals + 1, s));
   TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s,
              trax::launch_scan_u32(acc, acc, tiles_max + 1, acc + tiles_max + 1, 1, ctx->ctrl, ctx->status, s,
                                    scan_sub(ctx)));
-  TRAX_HIP_CHECK(ctx, trax::launch_acc_offsets(acc, slab, out->pair_offsets, out->totals + 0, pc, K + 1,
-                                               out->tracklet_offsets, out->totals + 2, s));
+  TRAX_HIP_CHECK(ctx, trax::launch_fused_offsets(tiles, ws->cand_counts, acc, slab, out->pair_offsets,
+                                                 out->totals + 0, pc, K + 1, out->cand_offsets,
+                                                 out->tracklet_offsets, out->totals + 1, s));
   TRAX_TIMED(ctx, TRAX_KERNEL_TRIPLET_PLACE, s,
              trax::launch_acc_place(sp, h, d_grid, pairs, out->pair_offsets, out->totals + 0, pc, acc,
                                     ws->cand_counts, slab, out->t1, out->t2, out->t3, out->tracklet_capacity,

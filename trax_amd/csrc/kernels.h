@@ -118,9 +118,10 @@ hipError_t launch_acc_redo(const SeedParams& sp, const SortedHits& h, const uint
                            const uint32_t* counts, uint4* acc_slab, uint32_t* acc_tiles, hipStream_t stream);
 hipError_t launch_acc_cut(const SeedParams& sp, const SortedHits& h, const uint32_t* d_npairs,
                           uint64_t pair_capacity, uint4* acc_slab, uint32_t* acc_tiles, hipStream_t stream);
-hipError_t launch_acc_offsets(const uint32_t* acc_scanned, const uint4* acc_slab, const uint32_t* pair_off,
-                              const uint32_t* d_npairs, uint64_t pair_capacity, uint32_t n, uint32_t* trk_off,
-                              uint32_t* d_total, hipStream_t stream);
+hipError_t launch_fused_offsets(const uint32_t* tile_scanned, const uint32_t* counts, const uint32_t* acc_scanned,
+                                const uint4* acc_slab, const uint32_t* pair_off, const uint32_t* d_npairs,
+                                uint64_t pair_capacity, uint32_t n, uint32_t* cand_off, uint32_t* trk_off,
+                                uint32_t* d_totals, hipStream_t stream);
 hipError_t launch_acc_place(const SeedParams& sp, const SortedHits& h, const uint32_t* grid, const uint2* pairs,
                             const uint32_t* pair_off, const uint32_t* d_npairs, uint64_t pair_capacity,
                             const uint32_t* acc_scanned, const uint32_t* counts, const uint4* acc_slab,

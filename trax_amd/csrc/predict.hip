@@ -865,36 +865,59 @@ __global__ __launch_bounds__(256) void k_acc_cut(SeedParams sp, SortedHits h, co
   }
 }
 
-// Triplet offsets per (event, layer triplet): the accepted records before
-// the segment's first pair, from the scanned tile totals plus, within that
-// pair's tile, the records of lower pairs (slab records are in pair order)
-// or the per-pair counts of an overflowed slab.  The last entry is the total.
-__global__ __launch_bounds__(256) void k_acc_offsets(const uint32_t* __restrict__ acc_scanned,
-                                                     const uint4* __restrict__ acc_slab,
-                                                     const uint32_t* __restrict__ pair_off,
-                                                     const uint32_t* __restrict__ d_npairs,
-                                                     uint64_t pair_capacity, uint32_t n,
-                                                     uint32_t* __restrict__ trk_off,
-                                                     uint32_t* __restrict__ d_total) {
+
+// Candidate and triplet offsets of the fused pass per (event, layer
+// triplet), one wave per segment: the scanned tile totals at the tile
+// holding the segment's first pair q, plus, within that tile, the slot
+// budgets of the pairs before q (one coalesced load and a reduction) and the
+// accepted records of pairs before q (their slab records, or the per-pair
+// counts of a tile with more than kAccSlots).  The last entries are the
+// totals.
+__global__ __launch_bounds__(256) void k_fused_offsets(const uint32_t* __restrict__ tile_scanned,
+                                                       const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ acc_scanned,
+                                                       const uint4* __restrict__ acc_slab,
+                                                       const uint32_t* __restrict__ pair_off,
+                                                       const uint32_t* __restrict__ d_npairs,
+                                                       uint64_t pair_capacity, uint32_t n,
+                                                       uint32_t* __restrict__ cand_off, uint32_t* __restrict__ trk_off,
+                                                       uint32_t* __restrict__ d_totals) {
+  const uint32_t lane = lane_id();
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < n; k += waves) {
     uint32_t q = pair_off[k];
     q = q < nP ? q : nP;
     const uint32_t t = q / kPredTile, lq = q - t * kPredTile;
-    uint32_t s = acc_scanned[t];
-    if (lq) {  // t is a tile with pairs
-      const uint32_t A = acc_scanned[t + 1u] - s;
+    // candidate slots of the tile-mates before q
+    const uint32_t j = t * kPredTile + lane;
+    uint32_t c = lane < lq ? counts[j] : 0u;
+    c = wave_reduce_sum(c);
+    // accepted records of the pairs before q
+    uint32_t a = 0;
+    if (lq) {
+      const uint32_t s0 = acc_scanned[t], A = acc_scanned[t + 1u] - s0;
       const uint4* slab = acc_slab + (uint64_t)t * kAccSlots;
       if (A <= kAccSlots) {
-        for (uint32_t j = 0; j < A && slab[j].x < q; ++j) ++s;
+        for (uint32_t r0 = 0; r0 < A; r0 += 64u) {
+          const uint32_t r = r0 + lane;
+          a += (uint32_t)__popcll(__ballot(r < A && slab[r].x < q));
+        }
       } else {
-        const uint32_t* c = reinterpret_cast<const uint32_t*>(slab);
-        for (uint32_t l = 0; l < lq; ++l) s += c[l];
+        const uint32_t v = lane < lq ? reinterpret_cast<const uint32_t*>(slab)[lane] : 0u;
+        a = wave_reduce_sum(v);
       }
     }
-    trk_off[k] = s;
-    if (k == n - 1u) *d_total = s;
+    if (lane == 0) {
+      const uint32_t co = tile_scanned[t] + c, to = acc_scanned[t] + a;
+      cand_off[k] = co;
+      trk_off[k] = to;
+      if (k == n - 1u) {
+        d_totals[0] = co;
+        d_totals[1] = to;
+      }
+    }
   }
 }
 
@@ -1046,13 +1069,14 @@ hipError_t launch_acc_cut(const SeedParams& sp, const SortedHits& h, const uint3
   return hipGetLastError();
 }
 
-hipError_t launch_acc_offsets(const uint32_t* acc_scanned, const uint4* acc_slab, const uint32_t* pair_off,
-                              const uint32_t* d_npairs, uint64_t pair_capacity, uint32_t n, uint32_t* trk_off,
-                              uint32_t* d_total, hipStream_t stream) {
-  uint32_t blocks = (n + 255u) / 256u;
-  if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_acc_offsets, dim3(blocks), dim3(256), 0, stream, acc_scanned, acc_slab, pair_off, d_npairs,
-                     pair_capacity, n, trk_off, d_total);
+hipError_t launch_fused_offsets(const uint32_t* tile_scanned, const uint32_t* counts, const uint32_t* acc_scanned,
+                                const uint4* acc_slab, const uint32_t* pair_off, const uint32_t* d_npairs,
+                                uint64_t pair_capacity, uint32_t n, uint32_t* cand_off, uint32_t* trk_off,
+                                uint32_t* d_totals, hipStream_t stream) {
+  uint32_t blocks = (n + 3u) / 4u;  // a wave per segment
+  blocks = blocks == 0 ? 1 : (blocks > 16384u ? 16384u : blocks);
+  hipLaunchKernelGGL(k_fused_offsets, dim3(blocks), dim3(256), 0, stream, tile_scanned, counts, acc_scanned,
+                     acc_slab, pair_off, d_npairs, pair_capacity, n, cand_off, trk_off, d_totals);
   return hipGetLastError();
 }
 

@@ -49,7 +49,8 @@ SHORT = [
     ("k_acc_redo", "acc_redo"),
     ("k_acc_cut", "acc_cut"),
     ("k_acc_place", "triplet_place"),
-    ("k_acc_offsets", "acc_offsets"),
+    ("k_acc_offsets", "acc_offsets"),  # earlier builds
+    ("k_fused_offsets", "fused_offsets"),
     ("k_predict_place", "predict_place"),
     ("k_predict<", "predict_two_phase"),
     ("k_filter_mask", "filter_mask"),
