@@ -472,11 +472,14 @@ hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const 
 // buffered candidates densely in the wave's LDS and runs the phi cut over
 // the queue.  The tile's phi survivors go to its slab as (lane | layer
 // triplet << 6, hit1, hit2, hit3) records in pair and slot order, their
-// number to the tile's accepted total (more than kAccSlots: the tile is
-// left to k_acc_redo, which walks it again with every cut and writes the
-// accepted records, or the per-pair accepted counts when there are more
-// than kAccSlots).  The theta and circle cuts then run over the survivors
-// of 64 tiles at a time (k_acc_cut), on full waves: a tile has ~14 phi
+// number to the tile's accepted total.  A tile with more phi survivors
+// than its slab holds (11 % of the tiles of the real ttbar events' coarse
+// 8 x 50 grids, none on PU200) runs the theta and circle cuts in the walk,
+// on its queue, and writes its accepted records as (pair, hit1, hit2, hit3)
+// (or the per-pair accepted counts when more than kAccSlots) and
+// kTileDone + its accepted total.  The theta and circle cuts of the other
+// tiles run over their survivors, 32 tiles at a time (k_acc_cut), on full
+// waves: a tile has ~14 phi
 // survivors and ~7 theta survivors on PU200, so in the walk each of those
 // cuts was a wave pass of mostly idle lanes (1.21 ms for the walk with all
 // three cuts, 1.06 ms with the phi cut, plus 0.09 ms for k_acc_cut).
@@ -486,6 +489,7 @@ hipError_t launch_predict_walk(const SeedParams& sp, const SortedHits& h, const 
 // the prefix sum of the accepted tile totals, pass 2 writes the triplets.
 constexpr uint32_t kAccSlots = kTileSlots / 4;  // uint4 records per tile slab
 constexpr uint32_t kTileRedo = 0xFFFFFFFFu;     // accepted total not known yet
+constexpr uint32_t kTileDone = 0x80000000u;     // + accepted total: finished in the walk
 
 // Accepted store-window hits of pair `it` among its slots [skip, budget),
 // in walk order (slots past the store window's hits hold no hit).  The rare
@@ -573,8 +577,9 @@ struct FuseLds {
 // One tile of pass 1 for the wave: pair q = t * 64 + lane.  One walk fills
 // the buffers and the slot budgets, the phi cut runs over the dense queue of
 // filled slots, and the tile's survivor total and survivor records are
-// written (kTileRedo when more than kAccSlots survive).  Returns false,
-// leaving the tile to k_acc_redo, when the spill pool overflows.
+// written (with more than kAccSlots survivors, the other two cuts run here
+// and the tile's accepted records are).  Returns false, leaving the tile to
+// k_acc_redo, when the spill pool overflows.
 template <uint32_t BUF>
 __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                            const uint2* pairs, const uint32_t* offs, uint32_t nP, uint32_t t,
@@ -622,6 +627,12 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   }
   L.acc[lane] = 0u;
   wave_lds_sync();
+  auto mark = [&](uint32_t ent) {
+    if (ent >= kSpillTag)
+      L.spill_acc[ent - kSpillTag] = 1;
+    else
+      atomicOr(&L.acc[ent & 63u], 1u << (ent >> 6));
+  };
   // the phi cut's pair angle, once per pair
   const float a1 = nq ? trax_atan2f((it.p2y - it.p1y), (it.p2x - it.p1x)) : 0.0f;
   queue_cut<false>(L.queue, nq, [&](uint32_t ent, bool live) {
@@ -635,28 +646,71 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
     const uint32_t i3 = spilled ? L.spill_hit[ent - kSpillTag] : L.buf[(tag >> 6) * 64u + l];
     const float4 c = load_xyzphi(h.xyzphi, i3);
     return filter_phi_a1(sp, lt, ang, x2, y2, c.x, c.y);
-  }, [&](uint32_t ent) {
-    if (ent >= kSpillTag)
-      L.spill_acc[ent - kSpillTag] = 1;
-    else
-      atomicOr(&L.acc[ent & 63u], 1u << (ent >> 6));
-  });
-  const uint32_t mask = L.acc[lane];
-  uint32_t a = (uint32_t)__popc(mask);
-  for (uint32_t k = 0; k < ns; ++k)  // spilled slots of this lane
-    a += (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) ? 1u : 0u;
-  const uint32_t aincl = wave_inclusive_scan(a);
-  const uint32_t A = __shfl(aincl, 63, 64);
-  if (lane == 0) acc_tiles[t] = A <= kAccSlots ? A : kTileRedo;
-  uint4* slab = acc_slab + (uint64_t)t * kAccSlots;
-  if (A <= kAccSlots && a) {
-    uint32_t pos = aincl - a;
-    const uint2 pr = pairs[q];  // read again: keeping it through the walk costs registers
-    const uint32_t tag = lane | (it.lt << 6);
-    for (uint32_t m = mask; m; m &= m - 1u)
-      slab[pos++] = make_uint4(tag, pr.x, pr.y, s_col[(uint32_t)__builtin_ctz(m) * 64u]);
+  }, mark);
+  auto marked_here = [&]() {  // marked slots of this lane
+    uint32_t n = (uint32_t)__popc(L.acc[lane]);
     for (uint32_t k = 0; k < ns; ++k)
-      if (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) slab[pos++] = make_uint4(tag, pr.x, pr.y, L.spill_hit[k]);
+      n += (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) ? 1u : 0u;
+    return n;
+  };
+  uint32_t a = marked_here();
+  uint32_t aincl = wave_inclusive_scan(a);
+  uint32_t A = __shfl(aincl, 63, 64);
+  const bool full = A > kAccSlots;
+  if (__builtin_expect(full, 0)) {
+    // the theta and circle cuts here; the pair's hits come from memory (its
+    // registers are not kept past the phi cut)
+    auto hits = [&](uint32_t ent, bool live, float4& p1, float4& p2, float4& p3) {
+      const bool spilled = ent >= kSpillTag;
+      const uint32_t tag = spilled ? (uint32_t)L.spill_tag[ent - kSpillTag] : ent;
+      const uint32_t l = tag & 63u;
+      const uint32_t lt = (uint32_t)__shfl(it.lt, (int)l, 64);
+      if (live) {
+        const uint32_t i3 = spilled ? L.spill_hit[ent - kSpillTag] : L.buf[(tag >> 6) * 64u + l];
+        const uint2 pr = pairs[t * kPredTile + l];
+        p1 = load_xyzphi(h.xyzphi, pr.x);
+        p2 = load_xyzphi(h.xyzphi, pr.y);
+        p3 = load_xyzphi(h.xyzphi, i3);
+      }
+      return lt;
+    };
+    // theta over the phi survivors (the marked queue entries), kept in the queue
+    const uint32_t n2 = queue_cut<true>(L.queue, nq, [&](uint32_t ent, bool live) {
+      const bool ok = live && (ent >= kSpillTag ? L.spill_acc[ent - kSpillTag] != 0
+                                                : ((L.acc[ent & 63u] >> (ent >> 6)) & 1u) != 0);
+      float4 p1, p2, p3;
+      const uint32_t lt = hits(ent, ok, p1, p2, p3);
+      return ok && filter_theta(sp, lt, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z);
+    }, [](uint32_t) {});
+    L.acc[lane] = 0u;
+    for (uint32_t k = lane; k < ns; k += 64u) L.spill_acc[k] = 0;
+    wave_lds_sync();
+    queue_cut<false>(L.queue, n2, [&](uint32_t ent, bool live) {
+      float4 p1, p2, p3;
+      const uint32_t lt = hits(ent, live, p1, p2, p3);
+      return live && filter_circle(sp, lt, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+    }, mark);
+    a = marked_here();
+    aincl = wave_inclusive_scan(a);
+    A = __shfl(aincl, 63, 64);
+  }
+  const uint32_t mask = L.acc[lane];
+  if (lane == 0) acc_tiles[t] = full ? kTileDone + A : A;
+  uint4* slab = acc_slab + (uint64_t)t * kAccSlots;
+  if (A <= kAccSlots) {
+    if (a) {
+      // phi survivors as (lane | layer triplet << 6, hit1, hit2, hit3), a
+      // finished tile's accepted records as (pair, hit1, hit2, hit3)
+      uint32_t pos = aincl - a;
+      const uint2 pr = pairs[q];  // read again: keeping it through the walk costs registers
+      const uint32_t tag = full ? q : (lane | (it.lt << 6));
+      for (uint32_t m = mask; m; m &= m - 1u)
+        slab[pos++] = make_uint4(tag, pr.x, pr.y, s_col[(uint32_t)__builtin_ctz(m) * 64u]);
+      for (uint32_t k = 0; k < ns; ++k)
+        if (L.spill_acc[k] && (L.spill_tag[k] & 63u) == lane) slab[pos++] = make_uint4(tag, pr.x, pr.y, L.spill_hit[k]);
+    }
+  } else {
+    reinterpret_cast<uint32_t*>(slab)[lane] = a;  // pass 2 walks the tile's pairs again
   }
   wave_lds_sync();  // the buffers, queue and pool are reused by the wave's next tile
   return true;
@@ -703,8 +757,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
 }
 
 // Tiles the walk marked kTileRedo (spill pool overflowed: more than kSpill
-// slots past the lane buffers; or more than kAccSlots phi survivors; none in
-// the PU200 and ttbar workloads): each lane walks its pair's slots one by one
+// slots past the lane buffers; none in the PU200 and ttbar workloads): each
+// lane walks its pair's slots one by one
 // with all three cuts, once to count and once to write the records (or the
 // per-pair counts).  It runs after k_acc_cut, which leaves these tiles alone.  One lane per tile finds the marked
 // tiles; the wave then takes them in turn.
@@ -749,8 +803,8 @@ __global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, co
   }
 }
 
-// Theta and circle cuts over the phi survivors of kCutTiles consecutive
-// tiles per block and round.  The tiles' survivor totals (written by the
+// Theta and circle cuts over the phi survivors of cut_group(tiles)
+// consecutive tiles per block and round.  The tiles' survivor totals (written by the
 // walk) are scanned in wave 0.  The survivors are then taken as one flat
 // range, in windows of whole tiles holding at most 256 * kCutPer records:
 // each thread reads kCutPer records (record r belongs to the tile whose
@@ -759,10 +813,18 @@ __global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, co
 // to its tile's slab as (pair, hit1, hit2, hit3) at the number of accepted
 // records before it in the tile (all of the window's records are read before
 // any is overwritten, and a record only moves down), with the tiles'
-// accepted totals.  Tiles the walk marked kTileRedo are left to k_acc_redo.
-constexpr uint32_t kCutTiles = 64;
+// accepted totals.  Tiles the walk finished (kTileDone + accepted total) get
+// their accepted total; tiles it marked kTileRedo are left to k_acc_redo.
 constexpr uint32_t kCutPer = 4;
 constexpr uint32_t kCutWindow = 256u * kCutPer;
+
+// Tiles per block and round of k_acc_cut: 32, or 16 below 65536 tiles, so
+// that a small batch still spreads over more than a thousand blocks (ttbar,
+// 19.5k tiles: 22 us at 16 against 35 us at 64; PU200, 485k tiles: 97 us at
+// 32, 100 us at 64, 108 us at 16).
+__host__ __device__ __forceinline__ uint32_t cut_group(uint64_t ntiles) {
+  return ntiles >= 65536u ? 32u : 16u;
+}
 
 // Set bits of bitmap w in [lo, hi).
 __device__ __forceinline__ uint32_t bits_in(const uint32_t* w, uint32_t lo, uint32_t hi) {
@@ -780,26 +842,28 @@ __device__ __forceinline__ uint32_t bits_in(const uint32_t* w, uint32_t lo, uint
 __global__ __launch_bounds__(256) void k_acc_cut(SeedParams sp, SortedHits h, const uint32_t* __restrict__ d_npairs,
                                                  uint64_t pair_capacity, uint4* __restrict__ acc_slab,
                                                  uint32_t* __restrict__ acc_tiles) {
-  __shared__ uint32_t s_incl[kCutTiles];
+  __shared__ uint32_t s_incl[64];
   __shared__ uint32_t s_ok[kCutWindow / 32u];
-  __shared__ unsigned long long s_redo;
+  __shared__ unsigned long long s_redo;  // tiles whose total is not the cut's (finished or redo)
   static_assert(kAccSlots <= kCutWindow, "a window holds a whole tile");
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, tid = threadIdx.x;
   const uint64_t nP64 = *d_npairs;
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
   const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
-  for (uint32_t t0 = blockIdx.x * kCutTiles; t0 < ntiles; t0 += gridDim.x * kCutTiles) {
+  const uint32_t G = cut_group(ntiles);
+  for (uint32_t t0 = blockIdx.x * G; t0 < ntiles; t0 += gridDim.x * G) {
     if (wave == 0) {
       const uint32_t t = t0 + lane;
-      const uint32_t v = t < ntiles ? acc_tiles[t] : 0u;
-      const bool redo = v == kTileRedo;
+      const uint32_t v = lane < G && t < ntiles ? acc_tiles[t] : 0u;
+      const bool redo = v >= kTileDone;  // kTileRedo, or finished in the walk
+      if (redo && v != kTileRedo) acc_tiles[t] = v - kTileDone;
       const uint32_t incl = wave_inclusive_scan(redo ? 0u : v);
       s_incl[lane] = incl;
       const uint64_t m = __ballot(redo);
       if (lane == 0) s_redo = m;
     }
     __syncthreads();
-    const uint32_t nt = ntiles - t0 < kCutTiles ? ntiles - t0 : kCutTiles;
+    const uint32_t nt = ntiles - t0 < G ? ntiles - t0 : G;
     for (uint32_t tb = 0; tb < nt;) {
       // window: tiles [tb, te] whose records fit after R0
       const uint32_t R0 = tb ? s_incl[tb - 1u] : 0u;
@@ -1060,9 +1124,11 @@ hipError_t launch_acc_redo(const SeedParams& sp, const SortedHits& h, const uint
 
 hipError_t launch_acc_cut(const SeedParams& sp, const SortedHits& h, const uint32_t* d_npairs,
                           uint64_t pair_capacity, uint4* acc_slab, uint32_t* acc_tiles, hipStream_t stream) {
-  // kCutTiles tiles per block and round
+  // cut_group(tiles) tiles per block and round (the kernel's group, from the
+  // actual tile count, is no larger)
   const uint64_t tiles = (pair_capacity + kPredTile - 1) / kPredTile;
-  uint64_t b = (tiles + kCutTiles - 1u) / kCutTiles;
+  const uint32_t g = cut_group(tiles);
+  uint64_t b = (tiles + g - 1u) / g;
   const uint32_t blocks = (uint32_t)(b == 0 ? 1 : (b > 16384u ? 16384u : b));
   hipLaunchKernelGGL(k_acc_cut, dim3(blocks), dim3(256), 0, stream, sp, h, d_npairs, pair_capacity, acc_slab,
                      acc_tiles);
