@@ -18,14 +18,14 @@ constexpr uint32_t kPredTile = 64;
 constexpr uint32_t kTileSlots = 512;
 
 uint64_t scan_tile_elems();
-// Scans of at least this many elements use the reduce-then-scan form.
+// Scans of at least this many elements use the two-pass form.
 constexpr uint64_t kReduceScanMin = 1ull << 21;
-// status: look-back words (single pass: one per tile; reduce-then-scan: one
+// status: look-back words (single pass: one per tile; two-pass: one
 // per chunk of 16384 elements), status_words of them.
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
                            uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
                            hipStream_t stream, int sub = 8, uint64_t status_words = kStatusWords);
-// Elements per chunk of the reduce-then-scan form (one status word each).
+// Elements per chunk of the two-pass form (one status word each).
 constexpr uint64_t kScanChunk = 16384;
 // Segmented fix-up after a global exclusive scan of data[0, n): bases[k] =
 // data[off[k]] (gather), then data[i] -= bases[segment of i].

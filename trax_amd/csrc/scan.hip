@@ -1,13 +1,14 @@
-// Exclusive uint32 prefix sum: single pass, decoupled look-back.
+// Exclusive uint32 prefix sum.
 //
 // Replaces the reference's three-kernel recursive Blelloch scan
 // (src/algorithms/PrefixSum.h:55-245, PrefixSum.cpp:60-126), which reads and
-// writes every element about twice per level.  Here each 256-thread block
-// owns one tile of SCAN_TILE elements, loaded and stored as coalesced uint4,
-// reduces it with wave64 shuffles, publishes the tile aggregate, and resolves
-// its exclusive prefix by looking back over the status words of its
-// predecessors (one wave, 64 predecessors per probe).  Traffic is one read
-// and one write per element (8 B).  The result is the exclusive scan mod 2^32,
+// writes every element about twice per level.  Below 2^21 elements: a single
+// pass, where each 256-thread block owns one tile of SCAN_TILE elements,
+// loaded and stored as coalesced uint4, reduces it with wave64 shuffles,
+// publishes the tile aggregate, and resolves its exclusive prefix by looking
+// back over the status words of its predecessors (one wave, 64 predecessors
+// per probe); one read and one write per element (8 B).  From 2^21 on: a
+// two-pass form (below).  The result is the exclusive scan mod 2^32,
 // identical to the reference's for every n (the reference's float launch
 // sizing, PrefixSum.cpp:37,66, can drop tail elements above 2^24; this one
 // does not).
@@ -15,6 +16,25 @@
 #include "kernels.h"
 
 namespace trax {
+
+__device__ __forceinline__ uint64_t scan_length(uint64_t n_max, const uint32_t* d_n, uint32_t n_add) {
+  if (!d_n) return n_max;
+  const uint64_t dn = (uint64_t)d_n[0] + n_add;
+  return dn < n_max ? dn : n_max;
+}
+
+__device__ __forceinline__ uint4 load4(const uint32_t* in, uint64_t idx, uint64_t n) {
+  uint4 v;
+  if (idx + 4 <= n) {
+    v = *reinterpret_cast<const uint4*>(in + idx);
+  } else {
+    v.x = idx < n ? in[idx] : 0u;
+    v.y = idx + 1 < n ? in[idx + 1] : 0u;
+    v.z = idx + 2 < n ? in[idx + 2] : 0u;
+    v.w = idx + 3 < n ? in[idx + 3] : 0u;
+  }
+  return v;
+}
 
 template <int SUB, bool PERSISTENT>
 __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
@@ -39,19 +59,18 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
     const uint32_t tile = s_tile, epoch = s_epoch;
     if (tile >= ntiles) break;
     const uint64_t base = (uint64_t)tile * TILE;
+    const bool whole = base + TILE <= n;
     uint4 v[SUB];
     uint32_t t[SUB], incl[SUB];
+    if (whole) {  // every load issued before the first use
+#pragma unroll
+      for (int s = 0; s < SUB; ++s) v[s] = *reinterpret_cast<const uint4*>(in + base + (uint64_t)s * 1024u + tid * 4u);
+    } else {
+#pragma unroll
+      for (int s = 0; s < SUB; ++s) v[s] = load4(in, base + (uint64_t)s * 1024u + tid * 4u, n);
+    }
 #pragma unroll
     for (int s = 0; s < SUB; ++s) {
-      const uint64_t idx = base + (uint64_t)s * 1024u + tid * 4u;
-      if (idx + 4 <= n) {
-        v[s] = *reinterpret_cast<const uint4*>(in + idx);
-      } else {
-        v[s].x = idx < n ? in[idx] : 0u;
-        v[s].y = idx + 1 < n ? in[idx + 1] : 0u;
-        v[s].z = idx + 2 < n ? in[idx + 2] : 0u;
-        v[s].w = idx + 3 < n ? in[idx + 3] : 0u;
-      }
       t[s] = v[s].x + v[s].y + v[s].z + v[s].w;
       incl[s] = wave_inclusive_scan(t[s]);
       if (lane == 63) s_wave[s][wave] = incl[s];
@@ -92,7 +111,7 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
       o.y = o.x + v[s].x;
       o.z = o.y + v[s].y;
       o.w = o.z + v[s].z;
-      if (idx + 4 <= n) {
+      if (whole || idx + 4 <= n) {
         *reinterpret_cast<uint4*>(out + idx) = o;
       } else {
         if (idx < n) out[idx] = o.x;
@@ -115,106 +134,88 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* in,
   }
 }
 
-// Reduce-then-scan form for large arrays (no atomics, no look-back): pass 1
-// reduces each chunk of kChunk elements to a partial sum, pass 2 scans the
-// partials in one block, pass 3 re-reads each chunk and writes its
-// exclusive scan.  12 B of HBM traffic per element instead of 8, but every
-// block runs independently at streaming bandwidth, and pass 3 visits the
-// chunks in reverse order so that the ones pass 1 read last are still in
-// the Infinity Cache.  The partials live in the look-back status words, one
-// 64-bit word each with a zero upper half: a later single-pass launch then
-// reads them as "not published" (state 0) and never as a tile of its own
-// epoch, whatever the partial's value.
+// Two-pass form for large arrays (no atomics, no look-back): the reduce pass
+// writes each chunk's sum; the scan pass re-reads each chunk, adds up the
+// sums of the chunks before it (every block of an XCD reads them from that
+// XCD's L2) and writes the chunk's exclusive scan.  12 B of HBM traffic per
+// element instead of 8, but every block runs independently at streaming
+// bandwidth, and the scan pass visits the chunks in reverse order so that
+// the ones the reduce pass read last are still in the Infinity Cache.  The
+// sums live in the look-back status words, one 64-bit word each with a zero
+// upper half: a later single-pass launch then reads them as "not published"
+// (state 0) and never as a tile of its own epoch, whatever the sum.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kChunkSub = 16;                     // uint4 per thread
 constexpr uint32_t kChunk = 256u * 4u * kChunkSub;    // elements per chunk
 
-__device__ __forceinline__ uint64_t scan_length(uint64_t n_max, const uint32_t* d_n, uint32_t n_add) {
-  if (!d_n) return n_max;
-  const uint64_t dn = (uint64_t)d_n[0] + n_add;
-  return dn < n_max ? dn : n_max;
-}
-
-__device__ __forceinline__ uint4 load4(const uint32_t* in, uint64_t idx, uint64_t n) {
-  uint4 v;
-  if (idx + 4 <= n) {
-    v = *reinterpret_cast<const uint4*>(in + idx);
-  } else {
-    v.x = idx < n ? in[idx] : 0u;
-    v.y = idx + 1 < n ? in[idx + 1] : 0u;
-    v.z = idx + 2 < n ? in[idx + 2] : 0u;
-    v.w = idx + 3 < n ? in[idx + 3] : 0u;
-  }
-  return v;
-}
-
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint64_t n_max,
-                                                     const uint32_t* __restrict__ d_n, uint32_t n_add,
-                                                     unsigned long long* __restrict__ partials) {
+                                                      const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                      unsigned long long* __restrict__ partials) {
   __shared__ uint32_t s_wave[4];
   const uint64_t n = scan_length(n_max, d_n, n_add);
-  const uint64_t base = (uint64_t)blockIdx.x * kChunk;
+  const uint32_t chunk = blockIdx.x;
+  const uint64_t base = (uint64_t)chunk * kChunk;
   if (base >= n) return;
   const uint32_t tid = threadIdx.x;
   uint32_t t = 0;
+  if (base + kChunk <= n) {  // every load issued before the first use
 #pragma unroll
-  for (uint32_t s = 0; s < kChunkSub; ++s) {
-    const uint4 v = load4(in, base + s * 1024u + tid * 4u, n);
-    t += v.x + v.y + v.z + v.w;
+    for (uint32_t s = 0; s < kChunkSub; ++s) {
+      const uint4 v = *reinterpret_cast<const uint4*>(in + base + s * 1024u + tid * 4u);
+      t += v.x + v.y + v.z + v.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t s = 0; s < kChunkSub; ++s) {
+      const uint4 v = load4(in, base + s * 1024u + tid * 4u, n);
+      t += v.x + v.y + v.z + v.w;
+    }
   }
   t = wave_reduce_sum(t);
   if (lane_id() == 0) s_wave[tid >> 6] = t;
   __syncthreads();
-  if (tid == 0) partials[blockIdx.x] = (unsigned long long)(s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3]);
-}
-
-// Exclusive scan of the chunk partials in place (one block of 1024 threads).
-__global__ __launch_bounds__(1024) void k_scan_partials(uint64_t n_max, const uint32_t* __restrict__ d_n,
-                                                        uint32_t n_add, unsigned long long* __restrict__ partials) {
-  __shared__ uint32_t s_wave[16];
-  const uint64_t n = scan_length(n_max, d_n, n_add);
-  const uint32_t np = (uint32_t)((n + kChunk - 1) / kChunk);
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-  uint32_t carry = 0;
-  for (uint32_t b0 = 0; b0 < np; b0 += 1024u) {
-    const uint32_t i = b0 + tid;
-    const uint32_t v = i < np ? (uint32_t)partials[i] : 0u;
-    const uint32_t incl = wave_inclusive_scan(v);
-    if (lane == 63) s_wave[wave] = incl;
-    __syncthreads();
-    uint32_t wp = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < 16; ++w) {
-      const uint32_t c = s_wave[w];
-      wp += (w < (int)wave) ? c : 0u;
-      tot += c;
-    }
-    if (i < np) partials[i] = (unsigned long long)(carry + wp + incl - v);
-    carry += tot;
-    __syncthreads();
-  }
+  if (tid == 0) partials[chunk] = (unsigned long long)(s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3]);
 }
 
 __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n_max,
-                                                   const uint32_t* __restrict__ d_n, uint32_t n_add,
-                                                   const unsigned long long* __restrict__ partials) {
+                                                    const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                    const unsigned long long* __restrict__ partials) {
   __shared__ uint32_t s_wave[kChunkSub][4];
+  __shared__ uint32_t s_pre[4];
   const uint64_t n = scan_length(n_max, d_n, n_add);
   const uint32_t chunk = gridDim.x - 1u - blockIdx.x;
   const uint64_t base = (uint64_t)chunk * kChunk;
   if (base >= n) return;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  const bool whole = base + kChunk <= n;
   uint4 v[kChunkSub];
+  if (whole) {
+#pragma unroll
+    for (uint32_t s = 0; s < kChunkSub; ++s) v[s] = *reinterpret_cast<const uint4*>(in + base + s * 1024u + tid * 4u);
+  } else {
+#pragma unroll
+    for (uint32_t s = 0; s < kChunkSub; ++s) v[s] = load4(in, base + s * 1024u + tid * 4u, n);
+  }
+  // the sums of chunks 0 .. chunk-1, 4096 per round
+  uint32_t pre = 0;
+  for (uint32_t b0 = 0; b0 < chunk; b0 += 4096u) {
+#pragma unroll
+    for (uint32_t r = 0; r < 16u; ++r) {
+      const uint32_t i = b0 + r * 256u + tid;
+      pre += i < chunk ? (uint32_t)partials[i] : 0u;
+    }
+  }
+  pre = wave_reduce_sum(pre);
+  if (lane == 0) s_pre[wave] = pre;
   uint32_t t[kChunkSub], incl[kChunkSub];
 #pragma unroll
   for (uint32_t s = 0; s < kChunkSub; ++s) {
-    v[s] = load4(in, base + s * 1024u + tid * 4u, n);
     t[s] = v[s].x + v[s].y + v[s].z + v[s].w;
     incl[s] = wave_inclusive_scan(t[s]);
     if (lane == 63) s_wave[s][wave] = incl[s];
   }
   __syncthreads();
-  uint32_t running = (uint32_t)partials[chunk];
+  uint32_t running = s_pre[0] + s_pre[1] + s_pre[2] + s_pre[3];
 #pragma unroll
   for (uint32_t s = 0; s < kChunkSub; ++s) {
     uint32_t wp = 0, tot = 0;
@@ -230,8 +231,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
     o.y = o.x + v[s].x;
     o.z = o.y + v[s].y;
     o.w = o.z + v[s].z;
-    if (idx + 4 <= n) {
-      // streaming store: the result is not read again by this scan
+    if (whole || idx + 4 <= n) {
       u32x4 ov = {o.x, o.y, o.z, o.w};
       __builtin_nontemporal_store(ov, reinterpret_cast<u32x4*>(out + idx));
     } else {
@@ -245,20 +245,16 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
 }
 
 // partials: ceil(n_max / kChunk) 64-bit words of scratch.
-static hipError_t scan_reduce_then_scan(const uint32_t* in, uint32_t* out, uint64_t n_max,
-                                        const uint32_t* d_n, uint32_t n_add, unsigned long long* partials,
-                                        hipStream_t stream) {
+static hipError_t scan_two_pass(const uint32_t* in, uint32_t* out, uint64_t n_max, const uint32_t* d_n,
+                                uint32_t n_add, unsigned long long* partials, hipStream_t stream) {
   const uint64_t blocks = (n_max + kChunk - 1) / kChunk;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)blocks), dim3(256), 0, stream, in, n_max, d_n, n_add,
-                     partials);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, stream, n_max, d_n, n_add, partials);
-  hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out, n_max, d_n, n_add,
-                     partials);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)blocks), dim3(256), 0, stream, in, n_max, d_n, n_add, partials);
+  hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out, n_max, d_n, n_add, partials);
   return hipGetLastError();
 }
 
-static_assert(kChunk == kScanChunk, "reduce-then-scan chunk");
+static_assert(kChunk == kScanChunk, "two-pass chunk");
 
 uint64_t scan_tile_elems() { return 256ull * 4ull * 8ull; }
 
@@ -342,9 +338,9 @@ hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n_max, co
                            uint32_t n_add, ScanCtrl* ctrl, unsigned long long* status,
                            hipStream_t stream, int sub, uint64_t status_words) {
   if (n_max >= kReduceScanMin) {
-    // the status words double as the partials of the three-pass form
+    // the status words hold the chunk sums of the two-pass form
     if ((n_max + kChunk - 1) / kChunk > status_words) return hipErrorInvalidValue;
-    return scan_reduce_then_scan(in, out, n_max, d_n, n_add, status, stream);
+    return scan_two_pass(in, out, n_max, d_n, n_add, status, stream);
   }
   // single pass: one status word per tile
   if ((n_max + 256ull * 4ull * sub - 1) / (256ull * 4ull * sub) > status_words) return hipErrorInvalidValue;

@@ -57,7 +57,6 @@ SHORT = [
     ("k_filter_store", "filter_store"),
     ("k_scan_u32", "scan"),
     ("k_scan_reduce", "scan_reduce"),
-    ("k_scan_partials", "scan_partials"),
     ("k_scan_down", "scan_down"),
     ("k_gather_offsets", "gather_offsets"),
     ("k_pred_offsets", "pred_offsets"),
