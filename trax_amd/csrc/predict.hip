@@ -601,15 +601,24 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
         L.spill_tag[k] = (uint16_t)((j << 6) | lane);
       }
     });
-    counts[q] = n_count;
   }
+  // The per-pair slot budgets are read back only for tiles holding a
+  // segment's first pair (the candidate offsets), the last tile (segments
+  // that start at the pair total) and tiles walked again (spill pool or slab
+  // overflow): the other tiles skip the 4 B per pair.
+  const bool seg_start = q < nP && offs[it.k] == q;
+  const bool keep = __ballot(seg_start) != 0ull || t == (nP - 1u) / kPredTile;
+  if (keep && q < nP) counts[q] = n_count;
   const uint32_t incl = wave_inclusive_scan(n_count);
   const uint32_t total = __shfl(incl, 63, 64);
   if (lane == 0) tile_counts[t] = total;
   const uint32_t n_eff = n_count < n_store ? n_count : n_store;  // slots [0, n_eff) hold a hit
   wave_lds_sync();
   const uint32_t ns = L.spill_n;
-  if (ns > kSpill) return false;
+  if (ns > kSpill) {
+    if (!keep && q < nP) counts[q] = n_count;
+    return false;
+  }
   // the queue: buffered slots, then pool entries within their pair's budget
   const uint32_t nb = n_eff < BUF ? n_eff : BUF;
   const uint32_t qincl = wave_inclusive_scan(nb);
@@ -657,6 +666,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   uint32_t aincl = wave_inclusive_scan(a);
   uint32_t A = __shfl(aincl, 63, 64);
   const bool full = A > kAccSlots;
+  if (full && !keep && q < nP) counts[q] = n_count;
   if (__builtin_expect(full, 0)) {
     // the theta and circle cuts here; the pair's hits come from memory (its
     // registers are not kept past the phi cut)
