@@ -98,10 +98,13 @@ ALGO_BYTES = {
     # mask words + scanned counts read; per triplet a candidate and a pair
     # read and three hit indexes written
     "filter_store": lambda c: 12 * c["cands"] // 64 + 28 * c["triplets"],
-    # fused prediction + filter walk: the walk's bytes without the stash
-    # slots, x, y, z of each candidate's third hit read, one 16-byte
-    # (pair, hit1, hit2, hit3) record per accepted triplet written
-    "predict_filter": lambda c: 16 * c["hits"] + 4 * c["cells"] + 12 * c["pairs"] + 12 * c["cands"]
+    # fused prediction + filter walk: one pair record read per pair (the
+    # per-pair slot budgets are written only for the few tiles read back),
+    # z and phi of every hit and the cells read, x, y, z of each candidate's
+    # third hit read, one 16-byte (pair, hit1, hit2, hit3) record per
+    # accepted triplet written (the phi survivors it hands to the cut kernel
+    # are intermediate records of the fused stage and not counted)
+    "predict_filter": lambda c: 16 * c["hits"] + 4 * c["cells"] + 8 * c["pairs"] + 12 * c["cands"]
                                 + 16 * c["triplets"],
     # per triplet its 16-byte record read and three hit indexes written
     "triplet_place": lambda c: 28 * c["triplets"],
@@ -136,7 +139,8 @@ def measured_traffic(workload: str, kernel: str):
 def measured_issue(workload: str, kernel: str):
     """VALU issue of `kernel` from the committed rocprofv3 SQ counter
     summary of this workload (SQ_ACTIVE_INST_VALU over the SIMDs' issue
-    slots, one wave64 VALU instruction per SIMD every 4 cycles), or None."""
+    slots at the measured 0.474 wave64 instructions per SIMD per cycle), or
+    None."""
     path = os.path.join(ROOT, "profiles", f"valu_issue_{workload}.md")
     if not os.path.exists(path):
         return None
@@ -414,7 +418,7 @@ def main():
     kern = {}
     reps = max(3, min(args.steps, 10))
     staged = ("predict_walk", "predict_place", "filter_mask", "filter_store")
-    fused = ("predict_filter", "triplet_place")
+    fused = ("predict_filter", "acc_cut", "triplet_place")
     for name in ("grid_build", "pairs") + staged + fused + ("scan",):
         b_, d_ = (fbufs, fdb) if name in fused else (bufs, db)
         ms_launch, n = ctx.time_kernel(name, lambda: runtime.seed_chain(ctx, b_, d_), reps)
@@ -577,6 +581,8 @@ def main():
                      "traffic_gbs": traffic / (dom["ms"] * 1e-3) / 1e9 if traffic else None,
                      "traffic_frac": traffic / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "ms": dom["ms"], "algorithmic_bytes": dom["bytes"],
+                     "candidates_per_sec": (n_cands / (dom["ms"] * 1e-3)
+                                            if dominant in ("predict_filter", "predict_walk") else None),
                      "limiter": limiter(dominant, issue, ta_busy, traffic / (dom["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
                                         if traffic else None),
                      "valu_issue": dict(issue, peak_wave_insts_per_simd_cycle=0.474,
