@@ -177,6 +177,11 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint64_
   if (tid == 0) partials[chunk] = (unsigned long long)(s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3]);
 }
 
+// NT: non-temporal stores, for a separate output (the input's lines then
+// stay in the Infinity Cache for the blocks still to come); a scan in place
+// writes the lines it has just read with ordinary stores (measured 121
+// against 128 us at 2^26 with non-temporal ones).
+template <bool NT>
 __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t* out, uint64_t n_max,
                                                     const uint32_t* __restrict__ d_n, uint32_t n_add,
                                                     const unsigned long long* __restrict__ partials) {
@@ -233,7 +238,10 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
     o.w = o.z + v[s].z;
     if (whole || idx + 4 <= n) {
       u32x4 ov = {o.x, o.y, o.z, o.w};
-      __builtin_nontemporal_store(ov, reinterpret_cast<u32x4*>(out + idx));
+      if (NT)
+        __builtin_nontemporal_store(ov, reinterpret_cast<u32x4*>(out + idx));
+      else
+        *reinterpret_cast<u32x4*>(out + idx) = ov;
     } else {
       if (idx < n) out[idx] = o.x;
       if (idx + 1 < n) out[idx + 1] = o.y;
@@ -250,7 +258,12 @@ static hipError_t scan_two_pass(const uint32_t* in, uint32_t* out, uint64_t n_ma
   const uint64_t blocks = (n_max + kChunk - 1) / kChunk;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)blocks), dim3(256), 0, stream, in, n_max, d_n, n_add, partials);
-  hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out, n_max, d_n, n_add, partials);
+  if (in == out)
+    hipLaunchKernelGGL(k_scan_down<false>, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out, n_max, d_n, n_add,
+                       partials);
+  else
+    hipLaunchKernelGGL(k_scan_down<true>, dim3((uint32_t)blocks), dim3(256), 0, stream, in, out, n_max, d_n, n_add,
+                       partials);
   return hipGetLastError();
 }
 
