@@ -439,9 +439,14 @@ def main():
         log(f"connectivity (tight, {conn['calls']} calls): {conn['ms_per_call']:.4f} ms/call, "
             f"{conn['gbs']:.0f} GB/s algorithmic")
 
-    # scan roofline on a large array (4 B read + 4 B write per element)
+    # scan roofline on a large array (4 B read + 4 B write per element), in
+    # place, on seeded random values: an all-zero array scans about 12 %
+    # faster on this part (zero lines are cheaper to move), which would
+    # flatter the figure
     n_scan = 1 << 26
-    big = torch.zeros(n_scan, dtype=torch.int32, device="cuda")
+    g_scan = torch.Generator(device="cuda").manual_seed(26)
+    big = torch.randint(-(1 << 31), (1 << 31) - 1, (n_scan,), dtype=torch.int32, device="cuda",
+                        generator=g_scan)
     scan_ms, _ = ctx.time_kernel("scan", lambda: runtime.prefix_sum(ctx, big), 10)
     scan_gbs = 8 * n_scan / (scan_ms * 1e-3) / 1e9
     grid_gbs = kern["grid_build"]["gbs"]
