@@ -228,3 +228,49 @@ def test_chain_wide_windows(ctx, sigma_phi, sigma_z):
     t = runtime.fetch(bufs, batch.n_events, batch.n_hits)
     assert np.array_equal(g["cands"], t["cands"])
     assert np.array_equal(co, t["cand_offsets"])
+
+
+def _event_without_layers(b1, layers):
+    """A one-event batch with the hits of `layers` (1-based) removed."""
+    from trax_amd import events as ev
+    keep = ~np.isin(b1.layer, np.asarray(layers, np.uint32))
+    nh = b1.layer_nhits.copy()
+    for l in layers:
+        nh[l - 1] = 0
+    off = np.concatenate([[0], np.cumsum(nh)[:-1]]).astype(np.uint32)
+    cols = [np.ascontiguousarray(c[keep]) for c in (b1.x, b1.y, b1.z, b1.layer, b1.det_id, b1.hit_id, b1.evt)]
+    return ev.EventBatch(*cols, np.zeros(1, np.uint32), nh.astype(np.uint32), off, b1.n_layers)
+
+
+def _empty_event(L):
+    from trax_amd import events as ev
+    return ev.EventBatch(*[np.zeros(0, np.float32)] * 3, *[np.zeros(0, np.uint32)] * 4, np.zeros(1, np.uint32),
+                         np.zeros(L, np.uint32), np.zeros(L, np.uint32), L)
+
+
+def test_chain_ragged_and_empty_events(ctx):
+    """A batch of uneven events: an empty event first and last, events with
+    their first, middle or outer layer empty, and an event with hits on one
+    layer only.  Every stage matches the oracle (segments of no pairs and
+    no triplets included), and the fused pass gives the staged triplets."""
+    from trax_amd import events as ev
+    from trax_amd import runtime
+    w = ev.config_workload(2, 4)
+    full = ev.make_batch(w)
+    L = w.seed_config.n_layers
+    parts = [_empty_event(L), full.event(0), _event_without_layers(full.event(1), [1]),
+             _event_without_layers(full.event(2), [2]), _empty_event(L),
+             _event_without_layers(full.event(3), [4]), _event_without_layers(full.event(0), [2, 3, 4]),
+             full.event(1), _empty_event(L)]
+    batch = ev.concat_events(parts)
+    g, o = _compare_chain(ctx, w.seed_config, batch)
+    P = len(w.seed_config.triplets)
+    toff = g["tracklet_offsets"]
+    assert toff[P] == 0 and toff[-1] - toff[-1 - P] == 0  # the empty events hold no triplets
+    assert len(o["tracklets"]) > 0
+    fbufs, _ = runtime.run_batch(ctx, w.seed_config, batch, candidates=False)
+    torch.cuda.synchronize()
+    f = runtime.fetch(fbufs, batch.n_events, batch.n_hits)
+    for k in ("tracklets", "tracklet_offsets", "cand_offsets"):
+        assert np.array_equal(f[k], g[k]), k
+    assert ctx.flags() == 0
