@@ -248,14 +248,17 @@ def _empty_event(L):
                          np.zeros(L, np.uint32), np.zeros(L, np.uint32), L)
 
 
-def test_chain_ragged_and_empty_events(ctx):
+@pytest.mark.parametrize("pair_generator", ["beamspot", "sector"])
+def test_chain_ragged_and_empty_events(ctx, pair_generator):
     """A batch of uneven events: an empty event first and last, events with
     their first, middle or outer layer empty, and an event with hits on one
     layer only.  Every stage matches the oracle (segments of no pairs and
-    no triplets included), and the fused pass gives the staged triplets."""
+    no triplets included), with either pair generator, and the fused pass
+    gives the staged triplets."""
     from trax_amd import events as ev
     from trax_amd import runtime
     w = ev.config_workload(2, 4)
+    w.seed_config.pair_generator = pair_generator
     full = ev.make_batch(w)
     L = w.seed_config.n_layers
     parts = [_empty_event(L), full.event(0), _event_without_layers(full.event(1), [1]),
