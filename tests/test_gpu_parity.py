@@ -230,6 +230,29 @@ def test_chain_wide_windows(ctx, sigma_phi, sigma_z):
     assert np.array_equal(co, t["cand_offsets"])
 
 
+def test_pair_store_past_the_count_masks(ctx):
+    """Coarse bins give pair windows of hundreds of visited hits.  The chain's
+    pair store takes the count pass's verdicts on each work item's first 64
+    visited hits and tests the rest itself; the separate count and store
+    entry points pass no verdicts.  Both must give the oracle's pairs."""
+    from trax_amd import events as ev
+    from trax_amd import runtime
+    w = ev.config_workload(3, 1)
+    w.seed_config.n_z, w.seed_config.n_phi = 16, 16
+    batch = ev.make_batch(w)
+    g, o = _compare_chain(ctx, w.seed_config, batch)
+    # an outer hit with more than 2 x 64 pairs has a work item past its mask
+    assert np.bincount(g["pairs"][:, 1]).max() > 128
+    bufs, db = runtime.run_batch(ctx, w.seed_config, batch)
+    runtime.pairs_count(ctx, bufs, db.n_events)
+    runtime.pairs_store(ctx, bufs, db.n_events)
+    torch.cuda.synchronize()
+    t = runtime.fetch(bufs, batch.n_events, batch.n_hits)
+    assert np.array_equal(g["pair_offsets"], t["pair_offsets"])
+    assert np.array_equal(g["pairs"], t["pairs"])
+    assert ctx.flags() == 0
+
+
 def _event_without_layers(b1, layers):
     """A one-event batch with the hits of `layers` (1-based) removed."""
     from trax_amd import events as ev

@@ -526,9 +526,21 @@ extern "C" int trax_grid_build(trax_ctx* ctx, const trax_seed_config* cfg, const
 }
 
 // -------------------------------------------------------------------- pairs
-extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
-                                const trax_hits* sorted, const uint32_t* d_grid,
-                                const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+// The predictor's stash is idle during the pair stage: when it holds one
+// 64-bit word per pair work item, count and store run back to back share
+// the count pass's verdicts through it (trax_pairs_beamspot).
+static unsigned long long* pair_masks(const trax_workspace* ws, uint32_t items_max) {
+  const uint64_t stash_bytes = (ws->pair_capacity + TRAX_PREDICT_TILE - 1) / TRAX_PREDICT_TILE *
+                               TRAX_PREDICT_TILE_SLOTS * 4ull;
+  if (!ws->cand_stash || (reinterpret_cast<uintptr_t>(ws->cand_stash) & 7u) ||
+      stash_bytes < (uint64_t)items_max * 8u)
+    return nullptr;
+  return reinterpret_cast<unsigned long long*>(ws->cand_stash);
+}
+
+static int pairs_count_impl(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                            const trax_hits* sorted, const uint32_t* d_grid, const trax_workspace* ws,
+                            trax_seed_outputs* out, void* stream, bool masks) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   trax::SeedParams sp;
@@ -542,8 +554,10 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   TRAX_HIP_CHECK(ctx, trax::launch_pair_plan(sp, d_grid, ws->work_offsets, ctx->ctrl, ctx->status, scan_sub(ctx),
                                              s));
   const uint32_t items_max = pair_items_max(sp, cfg, ws);
-  TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, nullptr, 0,
-                                         items_max, false, ctx->flags, ctx->launch.pairs_block, s));
+  unsigned long long* m = masks ? pair_masks(ws, items_max) : nullptr;
+  TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs(sp, h, d_grid, ws->work_offsets, ws->pair_counts, m,
+                                         nullptr, 0, items_max, false, ctx->flags, ctx->launch.pairs_block,
+                                         s));
   TRAX_TIMED(ctx, TRAX_KERNEL_SCAN, s, trax::launch_scan_u32(ws->pair_counts, ws->pair_counts, (uint64_t)items_max + 1,
                                             ws->work_offsets + K + 1, 1, ctx->ctrl, ctx->status, s,
                                             scan_sub(ctx)));
@@ -554,9 +568,9 @@ extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint
   return TRAX_OK;
 }
 
-extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
-                                const trax_hits* sorted, const uint32_t* d_grid,
-                                const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+static int pairs_store_impl(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                            const trax_hits* sorted, const uint32_t* d_grid, const trax_workspace* ws,
+                            trax_seed_outputs* out, void* stream, bool masks) {
   int rc = check_ctx(ctx);
   if (rc) return rc;
   trax::SeedParams sp;
@@ -565,11 +579,24 @@ extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint
   if (!sorted_ok(sorted) || !d_grid || !out || (!out->pairs && out->pair_capacity))
     return fail(ctx, TRAX_ERR_INVALID, "null pair-stage buffer");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t items_max = pair_items_max(sp, cfg, ws);
   TRAX_TIMED(ctx, TRAX_KERNEL_PAIRS, s, trax::launch_pairs(sp, sorted_view(sorted), d_grid, ws->work_offsets,
-                                         ws->pair_counts, reinterpret_cast<uint2*>(out->pairs),
-                                         out->pair_capacity, pair_items_max(sp, cfg, ws), true, ctx->flags,
-                                         ctx->launch.pairs_block, s));
+                                         ws->pair_counts, masks ? pair_masks(ws, items_max) : nullptr,
+                                         reinterpret_cast<uint2*>(out->pairs), out->pair_capacity, items_max,
+                                         true, ctx->flags, ctx->launch.pairs_block, s));
   return TRAX_OK;
+}
+
+extern "C" int trax_pairs_count(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                const trax_hits* sorted, const uint32_t* d_grid,
+                                const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  return pairs_count_impl(ctx, cfg, n_events, sorted, d_grid, ws, out, stream, false);
+}
+
+extern "C" int trax_pairs_store(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
+                                const trax_hits* sorted, const uint32_t* d_grid,
+                                const trax_workspace* ws, trax_seed_outputs* out, void* stream) {
+  return pairs_store_impl(ctx, cfg, n_events, sorted, d_grid, ws, out, stream, false);
 }
 
 extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, uint32_t n_events,
@@ -579,9 +606,9 @@ extern "C" int trax_pairs_beamspot(trax_ctx* ctx, const trax_seed_config* cfg, u
   // single pass with tile look-back, this is faster on large batches (PU200
   // 64 events: 0.445 vs 0.554 ms; 4096 ttbar events: 0.146 vs 0.245 ms): in
   // the single pass each tile's stores wait for its predecessors' counts.
-  int rc = trax_pairs_count(ctx, cfg, n_events, sorted, d_grid, ws, out, stream);
+  int rc = pairs_count_impl(ctx, cfg, n_events, sorted, d_grid, ws, out, stream, true);
   if (rc) return rc;
-  return trax_pairs_store(ctx, cfg, n_events, sorted, d_grid, ws, out, stream);
+  return pairs_store_impl(ctx, cfg, n_events, sorted, d_grid, ws, out, stream, true);
 }
 
 // ------------------------------------------------------------- sector pairs

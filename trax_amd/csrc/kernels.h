@@ -68,11 +68,13 @@ hipError_t launch_sector_pairs(const SeedParams& sp, const SortedHits& h, const 
                                hipStream_t stream);
 // Beam-spot pairs: at most this many work items per outer hit
 // (== TRAX_PAIR_PARTS, checked in the C ABI); the count pass also writes the
-// item count to work_off[K + 1].
+// item count to work_off[K + 1].  `masks` (one 64-bit word per work item,
+// or null) carries the count pass's verdicts on each item's first 64
+// visited hits to the store pass that follows it.
 constexpr uint32_t kPairParts = 2;
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                        uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
-                        uint32_t items_max, bool store, uint32_t* flags, uint32_t block,
+                        uint32_t* work_off, uint32_t* counts, unsigned long long* masks, uint2* pairs,
+                        uint64_t capacity, uint32_t items_max, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* scanned, uint32_t last, const uint32_t* last_dev,
                                  const uint32_t* idx, uint32_t n, uint32_t* out, uint32_t* d_total,

@@ -140,12 +140,18 @@ __global__ __launch_bounds__(256) void k_pair_counts(SeedParams sp, const uint32
 // row of the window, the phi cells [phiLowSector, phiHighSector), wrapping
 // at the row end (PairGeneratorBeamspot.h:172-230).  Every index is visited
 // once (the reference's bitmap collapses its repeated wrap-around visits).
+// f(idx, ord) gets each passing hit with its ordinal among the visited hits.
 template <typename F>
 __device__ __forceinline__ void pair_walk(const SeedParams& sp, const uint32_t* __restrict__ g1,
                                           const PairWindow& win, const SortedHits& h, F&& f) {
   const uint32_t nPhi = sp.n_phi;
   auto in_window = [&](float zi, float pi) {
     return win.zLow <= zi && zi <= win.zHigh && window_phi_ok(win.wrap, win.phiLow, win.phiHigh, pi);
+  };
+  uint32_t vis = 0;  // hits visited before the current range
+  auto scan_range = [&](uint32_t a, uint32_t b, const float2* __restrict__ zphi, auto&& test, auto&& g) {
+    trax::scan_range(a, b, zphi, test, [&](uint32_t idx) { return g(idx, vis + (idx - a)); });
+    vis += b > a ? b - a : 0u;
   };
   for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
     const uint32_t row = zs * (nPhi + 1);
@@ -220,11 +226,14 @@ __device__ __forceinline__ PairItem pair_work_item(const SeedParams& sp, const S
   return it;
 }
 
-// Count pass: one lane per work item walks its window.
+// Count pass: one lane per work item walks its window.  With `masks`, bit
+// j of masks[w] says whether the item's j-th visited hit (j < 64) passed,
+// so the store pass takes those hits without loading and testing them again.
 __global__ __launch_bounds__(512) void k_pairs_count(SeedParams sp, SortedHits h,
                                                      const uint32_t* __restrict__ grid,
                                                      uint32_t* __restrict__ work_off,
                                                      uint32_t* __restrict__ counts,
+                                                     unsigned long long* __restrict__ masks,
                                                      uint32_t items_max, uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
   const uint32_t K = sp.n_events * sp.n_triplets;
@@ -235,8 +244,14 @@ __global__ __launch_bounds__(512) void k_pairs_count(SeedParams sp, SortedHits h
   for (uint32_t w = sh.begin; w < sh.end; w += sh.stride) {
     const PairItem it = pair_work_item(sp, h, grid, offs, w);
     uint32_t n = 0;
-    pair_walk(sp, it.g1, it.win, h, [&](uint32_t) { ++n; return false; });
+    unsigned long long m = 0;
+    pair_walk(sp, it.g1, it.win, h, [&](uint32_t, uint32_t ord) {
+      m |= ord < 64u ? 1ull << ord : 0ull;
+      ++n;
+      return false;
+    });
     counts[w] = n;
+    if (masks) masks[w] = m;
   }
 }
 
@@ -258,12 +273,14 @@ struct StoreWindow {
   float zLow, zHigh, phiLow, phiHigh;
   uint32_t wrap, zLo, zHi, phiLoS, phiHiS, outer, pos;
   uint32_t g1;  // first cell of the inner layer's grid (offset into the grid)
+  uint32_t mlo, mhi, known;  // the count pass's mask of the first `known` visits
 };
 
 __global__ __launch_bounds__(256) void k_pairs_store(SeedParams sp, SortedHits h,
                                                      const uint32_t* __restrict__ grid,
                                                      const uint32_t* __restrict__ work_off,
                                                      const uint32_t* __restrict__ counts,  // scanned
+                                                     const unsigned long long* __restrict__ masks,
                                                      uint2* __restrict__ pairs, uint64_t capacity,
                                                      uint32_t items_max, uint32_t* __restrict__ flags) {
   extern __shared__ uint32_t s_offs[];  // K + 1 offsets when they fit (offsets_lds_bytes)
@@ -291,6 +308,10 @@ __global__ __launch_bounds__(256) void k_pairs_store(SeedParams sp, SortedHits h
       sw.outer = it.i;
       sw.pos = counts[w];
       sw.g1 = (uint32_t)(it.g1 - grid);
+      if (masks) {
+        const unsigned long long m = masks[w];
+        sw.mlo = (uint32_t)m, sw.mhi = (uint32_t)(m >> 32), sw.known = 64u;
+      }
     }
     wins[lane] = sw;
     wave_lds_sync();
@@ -299,14 +320,21 @@ __global__ __launch_bounds__(256) void k_pairs_store(SeedParams sp, SortedHits h
       const uint32_t* g1 = grid + pw.g1;
       const bool wrap = pw.wrap != 0u;
       uint64_t pos = pw.pos;
+      const unsigned long long known_mask = ((unsigned long long)pw.mhi << 32) | pw.mlo;
+      uint32_t vis = 0;  // hits visited before the current range
       // inner hits [lo, hi) in index order, kStoreGroup at a time
       auto range = [&](uint32_t lo, uint32_t hi) {
         for (uint32_t i0 = lo; i0 < hi; i0 += kStoreGroup) {
           const uint32_t idx = i0 + sub;
+          const uint32_t ord = vis + (idx - lo);
           bool ok = false;
           if (idx < hi) {
-            const float2 v = load_zphi(h.zphi, idx);
-            ok = pw.zLow <= v.x && v.x <= pw.zHigh && window_phi_ok(wrap, pw.phiLow, pw.phiHigh, v.y);
+            if (ord < pw.known) {
+              ok = (known_mask >> ord) & 1ull;
+            } else {
+              const float2 v = load_zphi(h.zphi, idx);
+              ok = pw.zLow <= v.x && v.x <= pw.zHigh && window_phi_ok(wrap, pw.phiLow, pw.phiHigh, v.y);
+            }
           }
           const uint32_t gb = (uint32_t)(__ballot(ok) >> (g * kStoreGroup)) & gmask;
           if (ok) {
@@ -318,6 +346,7 @@ __global__ __launch_bounds__(256) void k_pairs_store(SeedParams sp, SortedHits h
           }
           pos += (uint32_t)__popc(gb);
         }
+        vis += hi > lo ? hi - lo : 0u;
       };
       // the rows and ranges of pair_walk, in the same order
       for (uint32_t zs = pw.zLo; zs < pw.zHi; ++zs) {
@@ -389,19 +418,19 @@ hipError_t launch_pair_plan(const SeedParams& sp, const uint32_t* grid, uint32_t
 }
 
 hipError_t launch_pairs(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
-                        uint32_t* work_off, uint32_t* counts, uint2* pairs, uint64_t capacity,
-                        uint32_t items_max, bool store, uint32_t* flags, uint32_t block,
+                        uint32_t* work_off, uint32_t* counts, unsigned long long* masks, uint2* pairs,
+                        uint64_t capacity, uint32_t items_max, bool store, uint32_t* flags, uint32_t block,
                         hipStream_t stream) {
   if (store) {  // 256-thread blocks (the LDS windows are laid out for four waves)
     uint32_t blocks = (uint32_t)(((uint64_t)items_max + 255u) / 256u);
     blocks = xcd_grid(blocks == 0 ? 1u : (blocks > 16384u ? 16384u : blocks));
     hipLaunchKernelGGL(k_pairs_store, dim3(blocks), dim3(256), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
-                       counts, pairs, capacity, items_max, flags);
+                       counts, masks, pairs, capacity, items_max, flags);
   } else {
     uint32_t blocks = (uint32_t)(((uint64_t)items_max + block - 1) / block);
     blocks = xcd_grid(blocks == 0 ? 1u : (blocks > 16384u ? 16384u : blocks));
     hipLaunchKernelGGL(k_pairs_count, dim3(blocks), dim3(block), offsets_lds_bytes(sp), stream, sp, h, grid, work_off,
-                       counts, items_max, flags);
+                       counts, masks, items_max, flags);
   }
   return hipGetLastError();
 }
