@@ -282,7 +282,9 @@ int trax_pairs_store(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_even
 
 /* Single-call form: trax_pairs_count then trax_pairs_store with the total
  * kept on the device (no host read); writes out->pairs, out->pair_offsets and
- * out->totals[0].  Graph-capture safe. */
+ * out->totals[0].  Graph-capture safe.  When ws->cand_stash holds 8 bytes
+ * per pair work item, the count pass leaves its per-hit verdicts there for
+ * the store pass (overwriting the stash). */
 int trax_pairs_beamspot(trax_ctx *ctx, const trax_seed_config *cfg, uint32_t n_events,
                         const trax_hits *sorted, const uint32_t *d_grid, const trax_workspace *ws,
                         trax_seed_outputs *out, void *stream);
