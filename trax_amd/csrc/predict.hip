@@ -230,12 +230,14 @@ constexpr uint32_t kPredBuf = 16;
 constexpr uint32_t kSlotRewalk = 0xFFFFFFFEu;
 
 // The first BUF store-window hits go to s_col[j * STRIDE]; later ones to
-// spill(j, idx).
+// spill(j, idx).  With SCRATCH_ROW (the staged walk: 0.93 -> 0.90 ms on
+// PU200; the fused walk gains nothing from it and spills a register at its
+// tuned shape), s_col has BUF + 1 rows and row BUF is scratch.
 struct NoSpill {
   __device__ void operator()(uint32_t, uint32_t) const {}
 };
 
-template <uint32_t BUF = kPredBuf, uint32_t STRIDE = 256, typename Spill = NoSpill>
+template <uint32_t BUF = kPredBuf, uint32_t STRIDE = 256, bool SCRATCH_ROW = false, typename Spill = NoSpill>
 __device__ __forceinline__ uint32_t predict_count_buffer(const SeedParams& sp, const SortedHits& h,
                                                          const PredItem& it, uint32_t* s_col,
                                                          uint32_t* n_store_out, Spill&& spill = Spill{}) {
@@ -252,11 +254,26 @@ __device__ __forceinline__ uint32_t predict_count_buffer(const SeedParams& sp, c
   for (uint32_t zs = zlo; zs < zhi; ++zs) {
     const bool rc = zs >= wc.zLowSector && zs < wc.zHighSector;
     const bool rs = zs >= ws.zLowSector && zs < ws.zHighSector;
-    auto one = [&](uint32_t idx, float zi, float actPhi) {
+    // count-window and store-window verdicts of one hit
+    auto tests = [&](float zi, float actPhi, bool& in_s) {
       const bool ph = (!wrap && (phiLow <= actPhi && actPhi <= phiHigh)) ||
                       (wrap && ((phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= phiHigh)));
-      n_count += (rc && ph && czLo <= zi && zi <= czHi) ? 1u : 0u;
-      if (rs && ph && szLo <= zi && zi <= szHi) {
+      in_s = rs && ph && szLo <= zi && zi <= szHi;
+      return (rc && ph && czLo <= zi && zi <= czHi) ? 1u : 0u;
+    };
+    auto one = [&](uint32_t idx, float zi, float actPhi) {
+      bool in_s;
+      n_count += tests(zi, actPhi, in_s);
+      if (SCRATCH_ROW) {
+        // every visited hit is written to the lane's next free slot (row
+        // BUF, past the buffer, takes the writes beyond it), which only a
+        // store-window hit claims: no branch per hit while the buffer lasts
+        s_col[(n_store < BUF ? n_store : BUF) * STRIDE] = idx;
+        if (in_s) {
+          if (n_store >= BUF) spill(n_store, idx);
+          ++n_store;
+        }
+      } else if (in_s) {
         if (n_store < BUF)
           s_col[n_store * STRIDE] = idx;
         else
@@ -270,8 +287,28 @@ __device__ __forceinline__ uint32_t predict_count_buffer(const SeedParams& sp, c
         float2 v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = load_zphi(h.zphi, idx + u);
+        if (SCRATCH_ROW) {
+          // the four hits' slots; the hits past the buffer, if any, go to
+          // spill() in slot order afterwards
+          uint32_t sm = 0, p = n_store;  // sm: store-window hits of the four, one bit each
 #pragma unroll
-        for (int u = 0; u < 4; ++u) one(idx + u, v[u].x, v[u].y);
+          for (int u = 0; u < 4; ++u) {
+            bool in_s;
+            n_count += tests(v[u].x, v[u].y, in_s);
+            s_col[(p < BUF ? p : BUF) * STRIDE] = idx + u;
+            p += in_s ? 1u : 0u;
+            sm |= in_s ? 1u << u : 0u;
+          }
+          if (p > BUF) {
+            uint32_t j = n_store;
+            for (; sm; sm &= sm - 1u, ++j)
+              if (j >= BUF) spill(j, idx + (uint32_t)__builtin_ctz(sm));
+          }
+          n_store = p;
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) one(idx + u, v[u].x, v[u].y);
+        }
       }
       for (; idx < b; ++idx) {
         const float2 v = load_zphi(h.zphi, idx);
@@ -302,7 +339,7 @@ __global__ __launch_bounds__(256) void k_predict_walk(SeedParams sp, SortedHits 
                                                       uint32_t* __restrict__ slab,
                                                       uint32_t* __restrict__ tile_counts) {
   extern __shared__ uint32_t s_offs[];
-  __shared__ uint32_t s_buf[kPredBuf * 256];  // column per thread: s_buf[j * 256 + tid]
+  __shared__ uint32_t s_buf[(kPredBuf + 1) * 256];  // column per thread: s_buf[j * 256 + tid]
   const uint32_t K = sp.n_events * sp.n_triplets, tid = threadIdx.x, lane = lane_id();
   const uint32_t* offs = stage_offsets(pair_off, K + 1, s_offs);
   const uint64_t nP64 = *d_npairs;
@@ -315,7 +352,7 @@ __global__ __launch_bounds__(256) void k_predict_walk(SeedParams sp, SortedHits 
     uint32_t n_count = 0, n_store = 0;
     if (q < nP) {
       const PredItem it = pred_item(sp, h, grid, pairs, offs, q);
-      n_count = predict_count_buffer(sp, h, it, s_col, &n_store);
+      n_count = predict_count_buffer<kPredBuf, 256, true>(sp, h, it, s_col, &n_store);
       counts[q] = n_count;
     }
     const uint32_t incl = wave_inclusive_scan(n_count);
