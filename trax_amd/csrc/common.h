@@ -98,11 +98,14 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   return v;
 }
 
-__device__ __forceinline__ uint32_t wave_reduce_sum(uint32_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
+// Lane 63's value, read into a scalar register (the wave's total after an
+// inclusive scan).
+__device__ __forceinline__ uint32_t wave_last(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
+
+// Sum over the wave's 64 lanes (all lanes active), the same in every lane.
+__device__ __forceinline__ uint32_t wave_reduce_sum(uint32_t v) { return wave_last(wave_inclusive_scan(v)); }
 
 // Index k of the segment containing v: the last k with offs[k] <= v, for a
 // non-decreasing offs[0..n) with offs[0] <= v.

@@ -356,7 +356,7 @@ __global__ __launch_bounds__(256) void k_predict_walk(SeedParams sp, SortedHits 
       counts[q] = n_count;
     }
     const uint32_t incl = wave_inclusive_scan(n_count);
-    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t total = wave_last(incl);
     if (lane == 0) tile_counts[t] = total;
     if (total <= kTileSlots) {
       uint32_t* out = slab + (uint64_t)t * kTileSlots + (incl - n_count);
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(256) void k_predict_place(SeedParams sp, SortedHits
     const uint32_t incl = wave_inclusive_scan(n_own);
     const uint32_t base = tile_scanned[t];
     const uint32_t start = base + incl - n_own;  // first slot of pair q
-    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t total = wave_last(incl);
     if (total > kTileSlots) {  // slab overflowed: every pair of the tile walks again
       if (q < nP) {
         const uint32_t n = n_own;
@@ -647,7 +647,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   const bool keep = __ballot(seg_start) != 0ull || t == (nP - 1u) / kPredTile;
   if (keep && q < nP) counts[q] = n_count;
   const uint32_t incl = wave_inclusive_scan(n_count);
-  const uint32_t total = __shfl(incl, 63, 64);
+  const uint32_t total = wave_last(incl);
   if (lane == 0) tile_counts[t] = total;
   const uint32_t n_eff = n_count < n_store ? n_count : n_store;  // slots [0, n_eff) hold a hit
   wave_lds_sync();
@@ -659,7 +659,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   // the queue: buffered slots, then pool entries within their pair's budget
   const uint32_t nb = n_eff < BUF ? n_eff : BUF;
   const uint32_t qincl = wave_inclusive_scan(nb);
-  uint32_t nq = __shfl(qincl, 63, 64);
+  uint32_t nq = wave_last(qincl);
   for (uint32_t j = 0; j < nb; ++j) L.queue[qincl - nb + j] = (uint16_t)((j << 6) | lane);
   for (uint32_t k0 = 0; k0 < ns; k0 += 64u) {
     const uint32_t k = k0 + lane;
@@ -701,7 +701,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   };
   uint32_t a = marked_here();
   uint32_t aincl = wave_inclusive_scan(a);
-  uint32_t A = __shfl(aincl, 63, 64);
+  uint32_t A = wave_last(aincl);
   const bool full = A > kAccSlots;
   if (full && !keep && q < nP) counts[q] = n_count;
   if (__builtin_expect(full, 0)) {
@@ -739,7 +739,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
     }, mark);
     a = marked_here();
     aincl = wave_inclusive_scan(a);
-    A = __shfl(aincl, 63, 64);
+    A = wave_last(aincl);
   }
   const uint32_t mask = L.acc[lane];
   if (lane == 0) acc_tiles[t] = full ? kTileDone + A : A;
@@ -834,7 +834,7 @@ __global__ __launch_bounds__(64) void k_acc_redo(SeedParams sp, SortedHits h, co
         accept_walk(sp, h, it, 0u, n, [&](uint32_t) { ++a; });
       }
       const uint32_t aincl = wave_inclusive_scan(a);
-      const uint32_t A = __shfl(aincl, 63, 64);
+      const uint32_t A = wave_last(aincl);
       uint4* slab = acc_slab + (uint64_t)t * kAccSlots;
       if (A <= kAccSlots) {
         uint32_t pos = aincl - a;
@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(256) void k_acc_place(SeedParams sp, SortedHits h, 
     const bool big = A > kAccSlots;
     const uint32_t n = big ? 0u : A;
     const uint32_t incl = wave_inclusive_scan(n);
-    const uint32_t R = __shfl(incl, 63, 64);
+    const uint32_t R = wave_last(incl);
     if (lane < kPlaceTiles) {
       incl_w[lane] = incl;
       base_w[lane] = base;
