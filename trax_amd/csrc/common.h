@@ -107,6 +107,22 @@ __device__ __forceinline__ uint32_t wave_last(uint32_t v) {
 // Sum over the wave's 64 lanes (all lanes active), the same in every lane.
 __device__ __forceinline__ uint32_t wave_reduce_sum(uint32_t v) { return wave_last(wave_inclusive_scan(v)); }
 
+// Max (MAX true) or min over the wave's 64 lanes (all lanes active), the
+// same in every lane: the scan's moves, with lanes shifted in from outside
+// a row (and rows a broadcast does not write) reading the identity.
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_reduce_minmax(uint32_t v) {
+  const int id = MAX ? 0 : -1;
+  auto op = [](uint32_t a, int b) { const uint32_t u = (uint32_t)b; return MAX ? (u > a ? u : a) : (u < a ? u : a); };
+  v = op(v, __builtin_amdgcn_update_dpp(id, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = op(v, __builtin_amdgcn_update_dpp(id, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = op(v, __builtin_amdgcn_update_dpp(id, (int)v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = op(v, __builtin_amdgcn_update_dpp(id, (int)v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = op(v, __builtin_amdgcn_update_dpp(id, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = op(v, __builtin_amdgcn_update_dpp(id, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return wave_last(v);
+}
+
 // Index k of the segment containing v: the last k with offs[k] <= v, for a
 // non-decreasing offs[0..n) with offs[0] <= v.
 __device__ __forceinline__ uint32_t segment_of(const uint32_t* __restrict__ offs, uint32_t n,

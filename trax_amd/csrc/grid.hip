@@ -189,12 +189,8 @@ template <int NW>
 __device__ __forceinline__ void block_min_max(uint32_t lo, uint32_t hi, uint32_t* s_wave,
                                               uint32_t* out_lo, uint32_t* out_hi) {
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    const uint32_t l = __shfl_xor(lo, d, 64), h = __shfl_xor(hi, d, 64);
-    lo = l < lo ? l : lo;
-    hi = h > hi ? h : hi;
-  }
+  lo = wave_reduce_minmax<false>(lo);
+  hi = wave_reduce_minmax<true>(hi);
   if (lane == 0) {
     s_wave[wave] = lo;
     s_wave[NW + wave] = hi;
@@ -286,11 +282,7 @@ __device__ __forceinline__ uint32_t block_scan_occupancy(uint32_t* cnt, uint32_t
     carry += tot;
     __syncthreads();
   }
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    const uint32_t o = __shfl_xor(occ, d, 64);
-    occ = o > occ ? o : occ;
-  }
+  occ = wave_reduce_minmax<true>(occ);
   if (lane == 0) s_wave[NW + wave] = occ;
   __syncthreads();
   occ = 0;
