@@ -231,8 +231,9 @@ constexpr uint32_t kSlotRewalk = 0xFFFFFFFEu;
 
 // The first BUF store-window hits go to s_col[j * STRIDE]; later ones to
 // spill(j, idx).  With SCRATCH_ROW (the staged walk: 0.93 -> 0.90 ms on
-// PU200; the fused walk gains nothing from it and spills a register at its
-// tuned shape), s_col has BUF + 1 rows and row BUF is scratch.
+// PU200; in the fused walk the LDS write per visit costs more than the
+// branch it removes, 1.12 against 1.03 ms), s_col has BUF + 1 rows and row
+// BUF is scratch.
 struct NoSpill {
   __device__ void operator()(uint32_t, uint32_t) const {}
 };
