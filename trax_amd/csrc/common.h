@@ -326,9 +326,13 @@ __device__ __forceinline__ bool scan_range(uint32_t a, uint32_t b, const float2*
     for (int u = 0; u < 4; ++u)
       if (in_window(v[u].x, v[u].y) && f(idx + u)) return true;
   }
-  for (; idx < b; ++idx) {
-    const float2 v = load_zphi(zphi, idx);
-    if (in_window(v.x, v.y) && f(idx)) return true;
+  if (idx < b) {  // the last one to three hits, loaded together
+    float2 v[3];
+#pragma unroll
+    for (uint32_t u = 0; u < 3; ++u) v[u] = load_zphi(zphi, idx + u < b ? idx + u : idx);
+#pragma unroll
+    for (uint32_t u = 0; u < 3; ++u)
+      if (idx + u < b && in_window(v[u].x, v[u].y) && f(idx + u)) return true;
   }
   return false;
 }

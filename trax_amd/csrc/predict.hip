@@ -311,9 +311,13 @@ __device__ __forceinline__ uint32_t predict_count_buffer(const SeedParams& sp, c
           for (int u = 0; u < 4; ++u) one(idx + u, v[u].x, v[u].y);
         }
       }
-      for (; idx < b; ++idx) {
-        const float2 v = load_zphi(h.zphi, idx);
-        one(idx, v.x, v.y);
+      if (idx < b) {  // the last one to three hits, loaded together
+        float2 v[3];
+#pragma unroll
+        for (uint32_t u = 0; u < 3; ++u) v[u] = load_zphi(h.zphi, idx + u < b ? idx + u : idx);
+#pragma unroll
+        for (uint32_t u = 0; u < 3; ++u)
+          if (idx + u < b) one(idx + u, v[u].x, v[u].y);
       }
     };
     const uint32_t row = zs * (nPhi + 1);
