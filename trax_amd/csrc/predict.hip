@@ -123,19 +123,17 @@ struct PredItem {
   float p1x, p1y, p1z, phi1, p2x, p2y, p2z, phi2, minR, maxR, acosMinR, acosMaxR;
 };
 
-__device__ __forceinline__ PredItem pred_item(const SeedParams& sp, const SortedHits& h,
-                                              const uint32_t* __restrict__ grid,
-                                              const uint2* __restrict__ pairs,
-                                              const uint32_t* offs, uint32_t q) {
+// Pair q's item from its hits' (x, y, z, phi), a = inner, b = outer.
+__device__ __forceinline__ PredItem pred_item_from(const SeedParams& sp, const uint32_t* __restrict__ grid,
+                                                   const uint32_t* offs, uint32_t q, const float4& a,
+                                                   const float4& b) {
   const uint32_t P = sp.n_triplets, K = sp.n_events * P, C = sp.cells, L = sp.n_layers;
   PredItem it;
   it.k = segment_of(offs, K + 1, q);
   const uint32_t e = it.k / P;
   it.lt = it.k - e * P;
-  const uint2 pr = pairs[q];
   const uint32_t layer = sp.layer3[it.lt] - 1;
   it.g3 = grid + ((uint64_t)e * L + layer) * C;
-  const float4 a = load_xyzphi(h.xyzphi, pr.x), b = load_xyzphi(h.xyzphi, pr.y);
   it.p1x = a.x, it.p1y = a.y, it.p1z = a.z, it.phi1 = a.w;
   it.p2x = b.x, it.p2y = b.y, it.p2z = b.z, it.phi2 = b.w;
   it.minR = sp.min_radius[layer];
@@ -143,6 +141,14 @@ __device__ __forceinline__ PredItem pred_item(const SeedParams& sp, const Sorted
   it.acosMinR = sp.acos_min_r[layer];
   it.acosMaxR = sp.acos_max_r[layer];
   return it;
+}
+
+__device__ __forceinline__ PredItem pred_item(const SeedParams& sp, const SortedHits& h,
+                                              const uint32_t* __restrict__ grid,
+                                              const uint2* __restrict__ pairs,
+                                              const uint32_t* offs, uint32_t q) {
+  const uint2 pr = pairs[q];
+  return pred_item_from(sp, grid, offs, q, load_xyzphi(h.xyzphi, pr.x), load_xyzphi(h.xyzphi, pr.y));
 }
 
 template <bool STORE>
@@ -622,11 +628,12 @@ struct FuseLds {
 // written (with more than kAccSlots survivors, the other two cuts run here
 // and the tile's accepted records are).  Returns false, leaving the tile to
 // k_acc_redo, when the spill pool overflows.
-template <uint32_t BUF>
+template <uint32_t BUF, typename Next>
 __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHits& h, const uint32_t* grid,
                                            const uint2* pairs, const uint32_t* offs, uint32_t nP, uint32_t t,
                                            FuseLds<BUF>& L, uint32_t* counts, uint32_t* tile_counts, uint4* acc_slab,
-                                           uint32_t* acc_tiles) {
+                                           uint32_t* acc_tiles, const float4& hit1, const float4& hit2,
+                                           Next&& next) {
   const uint32_t lane = lane_id();
   const uint32_t q = t * kPredTile + lane;
   uint32_t* s_col = L.buf + lane;
@@ -635,7 +642,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
   uint32_t n_count = 0, n_store = 0;
   PredItem it{};
   if (q < nP) {
-    it = pred_item(sp, h, grid, pairs, offs, q);
+    it = pred_item_from(sp, grid, offs, q, hit1, hit2);
     n_count = predict_count_buffer<BUF, 64>(sp, h, it, s_col, &n_store, [&](uint32_t j, uint32_t idx) {
       const uint32_t k = atomicAdd(&L.spill_n, 1u);
       if (k < kSpill) {
@@ -644,6 +651,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
       }
     });
   }
+  next();  // the next tile's loads start here, behind this tile's cuts
   // The per-pair slot budgets are read back only for tiles holding a
   // segment's first pair (the candidate offsets), the last tile (segments
   // that start at the pair total) and tiles walked again (spill pool or slab
@@ -792,9 +800,31 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE))) 
   const uint32_t nP = (uint32_t)(nP64 < pair_capacity ? nP64 : pair_capacity);
   const uint32_t ntiles = (nP + kPredTile - 1) / kPredTile;
   const XcdShare sh = xcd_share(ntiles, NW, wave);
+  // Each tile's pair records are loaded a tile ahead and their hits during
+  // the tile before (after its walk), so a tile's walk starts on resident
+  // data (fused walk 1.005 -> 0.994 ms on PU200).
+  const uint32_t lane = lane_id();
+  auto has_pair = [&](uint32_t tt) { return tt < sh.end && tt * kPredTile + lane < nP; };
+  uint2 pr_next = make_uint2(0u, 0u);
+  float4 a_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b_next = a_next;
+  if (has_pair(sh.begin)) {
+    pr_next = pairs[sh.begin * kPredTile + lane];
+    a_next = load_xyzphi(h.xyzphi, pr_next.x);
+    b_next = load_xyzphi(h.xyzphi, pr_next.y);
+  }
   for (uint32_t t = sh.begin; t < sh.end; t += sh.stride) {
-    if (!fused_tile<BUF>(sp, h, grid, pairs, offs, nP, t, s_lds[wave], counts, tile_counts, acc_slab, acc_tiles) &&
-        lane_id() == 0)
+    const float4 a = a_next, b = b_next;
+    const uint32_t t2 = t + sh.stride;
+    const bool more = has_pair(t2);
+    if (more) pr_next = pairs[t2 * kPredTile + lane];
+    if (!fused_tile<BUF>(sp, h, grid, pairs, offs, nP, t, s_lds[wave], counts, tile_counts, acc_slab, acc_tiles, a, b,
+                         [&]() {
+                           if (more) {
+                             a_next = load_xyzphi(h.xyzphi, pr_next.x);
+                             b_next = load_xyzphi(h.xyzphi, pr_next.y);
+                           }
+                         }) &&
+        lane == 0)
       acc_tiles[t] = kTileRedo;  // left to k_acc_redo
   }
   // a zero after the last tile and the tile count (the scans' length) past
