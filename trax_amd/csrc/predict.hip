@@ -652,6 +652,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
     });
   }
   next();  // the next tile's loads start here, behind this tile's cuts
+  const uint2 pr_q = q < nP ? pairs[q] : make_uint2(0u, 0u);  // for the slab records, behind the cuts
   // The per-pair slot budgets are read back only for tiles holding a
   // segment's first pair (the candidate offsets), the last tile (segments
   // that start at the pair total) and tiles walked again (spill pool or slab
@@ -762,7 +763,7 @@ __device__ __forceinline__ bool fused_tile(const SeedParams& sp, const SortedHit
       // phi survivors as (lane | layer triplet << 6, hit1, hit2, hit3), a
       // finished tile's accepted records as (pair, hit1, hit2, hit3)
       uint32_t pos = aincl - a;
-      const uint2 pr = pairs[q];  // read again: keeping it through the walk costs registers
+      const uint2 pr = pr_q;
       const uint32_t tag = full ? q : (lane | (it.lt << 6));
       for (uint32_t m = mask; m; m &= m - 1u)
         slab[pos++] = make_uint4(tag, pr.x, pr.y, s_col[(uint32_t)__builtin_ctz(m) * 64u]);
