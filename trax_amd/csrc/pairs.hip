@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_pair_counts(SeedParams sp, const uint32
 // row of the window, the phi cells [phiLowSector, phiHighSector), wrapping
 // at the row end (PairGeneratorBeamspot.h:172-230).  Every index is visited
 // once (the reference's bitmap collapses its repeated wrap-around visits).
-// f(idx, ord) gets each passing hit with its ordinal among the visited hits.
+// f(bits, ord) gets the visits' verdicts in groups (see scan_range below).
 template <typename F>
 __device__ __forceinline__ void pair_walk(const SeedParams& sp, const uint32_t* __restrict__ g1,
                                           const PairWindow& win, const SortedHits& h, F&& f) {
@@ -149,8 +149,28 @@ __device__ __forceinline__ void pair_walk(const SeedParams& sp, const uint32_t* 
     return win.zLow <= zi && zi <= win.zHigh && window_phi_ok(win.wrap, win.phiLow, win.phiHigh, pi);
   };
   uint32_t vis = 0;  // hits visited before the current range
+  // f(bits, ord): the verdicts of up to four consecutive visits, bit u for
+  // the visit with ordinal ord + u
   auto scan_range = [&](uint32_t a, uint32_t b, const float2* __restrict__ zphi, auto&& test, auto&& g) {
-    trax::scan_range(a, b, zphi, test, [&](uint32_t idx) { return g(idx, vis + (idx - a)); });
+    uint32_t idx = a;
+    for (; idx + 4 <= b; idx += 4) {
+      float2 v[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) v[u] = load_zphi(zphi, idx + u);
+      uint32_t bits = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) bits |= test(v[u].x, v[u].y) ? 1u << u : 0u;
+      g(bits, vis + (idx - a));
+    }
+    if (idx < b) {  // the last one to three hits, loaded together
+      float2 v[3];
+#pragma unroll
+      for (uint32_t u = 0; u < 3; ++u) v[u] = load_zphi(zphi, idx + u < b ? idx + u : idx);
+      uint32_t bits = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < 3; ++u) bits |= idx + u < b && test(v[u].x, v[u].y) ? 1u << u : 0u;
+      g(bits, vis + (idx - a));
+    }
     vis += b > a ? b - a : 0u;
   };
   for (uint32_t zs = win.zLowSector; zs < win.zHighSector; ++zs) {
@@ -245,10 +265,9 @@ __global__ __launch_bounds__(512) void k_pairs_count(SeedParams sp, SortedHits h
     const PairItem it = pair_work_item(sp, h, grid, offs, w);
     uint32_t n = 0;
     unsigned long long m = 0;
-    pair_walk(sp, it.g1, it.win, h, [&](uint32_t, uint32_t ord) {
-      m |= ord < 64u ? 1ull << ord : 0ull;
-      ++n;
-      return false;
+    pair_walk(sp, it.g1, it.win, h, [&](uint32_t bits, uint32_t ord) {
+      m |= ord < 64u ? (unsigned long long)bits << ord : 0ull;
+      n += (uint32_t)__popc(bits);
     });
     counts[w] = n;
     if (masks) masks[w] = m;
