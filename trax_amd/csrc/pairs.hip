@@ -281,7 +281,7 @@ __global__ __launch_bounds__(512) void k_pairs_count(SeedParams sp, SortedHits h
 // (claimed rather than four fixed items per group: 15 us less on PU200, the
 // groups that draw short walks take more items).  A ballot ranks the group's
 // passing hits, so the records go out in walk order as contiguous runs
-// (measured HBM write traffic 1.27x the records, against 2.4x when every
+// (measured HBM write traffic 1.12x the records, against 2.4x when every
 // lane wrote its own run: partially written lines were evicted and written
 // again) and the (z, phi) loads cover consecutive hits.  Each lane first
 // evaluates its own item's window and parks it in LDS for its group.
